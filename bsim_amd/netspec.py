@@ -1,0 +1,304 @@
+"""Flat description of a BSim network and seeded generators for the workloads.
+
+A :class:`NetSpec` is the network exactly as BSim's ``Network`` would hold it
+after the user's ``createPopulation``/``connect`` calls: neurons in creation
+order (population after population), synapses in ``connect`` call order.  Real
+values are kept in float64 as written by the user; a float build rounds them
+to float32 once (``(float)x``), exactly as passing a double literal to a
+``real`` parameter does in the reference.
+
+Per-neuron LIF row (``LIF_NPARAM`` values), the arguments of
+``LIF_brian(LIFENeuron(v_init, v_rest, v_reset, cm, tau_m, tau_refrac,
+tau_syn_E, tau_syn_I, v_thresh, i_offset), tau_syn_E, tau_syn_I)``
+(src/neuron/lif/LIFENeuron.h:9, src/base/CompositeNeuron.h:160):
+
+    0 v_init  1 v_rest  2 v_reset  3 cm  4 tau_m  5 tau_refrac
+    6 tau_syn_E  7 tau_syn_I  8 v_thresh  9 i_offset
+    10 composite tau_syn_E  11 composite tau_syn_I
+
+Neuron kinds: ``LIF`` (LIF_brian), ``ARRAY`` (Array_spikes: fires at the
+listed cycles, src/neuron/array/ArrayNeuron.cpp:115-128) and ``BERNOULLI``
+(a seeded per-cycle Bernoulli source, this build's deterministic stand-in for
+the reference's clock-seeded Poisson_spikes, src/neuron/poisson/
+PoissonNeuron.cpp:25-26).
+"""
+from __future__ import annotations
+
+from dataclasses import dataclass, field
+
+import numpy as np
+
+LIF, ARRAY, BERNOULLI = 0, 1, 2
+LIF_NPARAM = 12
+
+
+@dataclass
+class NetSpec:
+    kind: np.ndarray                 # int8 [n]
+    lif: np.ndarray                  # float64 [n, 12]
+    pre: np.ndarray                  # int32 [s]
+    post: np.ndarray                 # int32 [s]
+    w: np.ndarray                    # float32/64 [s]
+    d: np.ndarray                    # float32/64 [s] (seconds)
+    stype: np.ndarray | None = None  # int8 [s], 0 excitatory / 1 inhibitory
+    tau: np.ndarray | None = None    # [s], 0 -> destination's composite tau
+    arr_off: np.ndarray | None = None  # int64 [n+1]
+    arr_t: np.ndarray | None = None    # int32 spike cycles
+    bern_p: np.ndarray | None = None   # uint32 [n] threshold of the source hash
+    seed: int = 0
+    pops: list = field(default_factory=list)  # [(start, count)] per population
+
+    @property
+    def n(self) -> int:
+        return int(len(self.kind))
+
+    @property
+    def s(self) -> int:
+        return int(len(self.pre))
+
+
+def lif_row(v_init, v_rest, v_reset, cm, tau_m, tau_refrac, tau_syn_E, tau_syn_I,
+            v_thresh, i_offset, comp_tau_E=None, comp_tau_I=None):
+    if comp_tau_E is None:
+        comp_tau_E = tau_syn_E
+    if comp_tau_I is None:
+        comp_tau_I = tau_syn_I
+    return np.array([v_init, v_rest, v_reset, cm, tau_m, tau_refrac, tau_syn_E, tau_syn_I,
+                     v_thresh, i_offset, comp_tau_E, comp_tau_I], np.float64)
+
+
+class Builder:
+    """Incremental construction in the reference's call order."""
+
+    def __init__(self):
+        self.kinds = []
+        self.lifs = []
+        self.arr = []           # per neuron list of cycles (ARRAY) or None
+        self.bern = []          # per neuron threshold
+        self.pre, self.post, self.w, self.d, self.stype, self.tau = [], [], [], [], [], []
+        self.pops = []
+        self.n = 0
+
+    def population(self, rows, kind=LIF, spikes=None, bern_p=None):
+        rows = np.atleast_2d(np.asarray(rows, np.float64))
+        cnt = rows.shape[0]
+        start = self.n
+        self.kinds.append(np.full(cnt, kind, np.int8))
+        self.lifs.append(rows)
+        for i in range(cnt):
+            self.arr.append(None if spikes is None else np.asarray(spikes[i], np.int32))
+        self.bern.append(np.zeros(cnt, np.uint32) if bern_p is None
+                         else np.broadcast_to(np.asarray(bern_p, np.uint32), (cnt,)).copy())
+        self.pops.append((start, cnt))
+        self.n += cnt
+        return len(self.pops) - 1
+
+    def connect(self, pre, post, w, d, stype=None, tau=None):
+        pre = np.asarray(pre, np.int32)
+        self.pre.append(pre)
+        self.post.append(np.asarray(post, np.int32))
+        self.w.append(np.asarray(w))
+        self.d.append(np.asarray(d))
+        self.stype.append(np.zeros(len(pre), np.int8) if stype is None
+                          else np.asarray(stype, np.int8))
+        self.tau.append(np.zeros(len(pre)) if tau is None else np.asarray(tau))
+
+    def connect_all_to_all(self, psrc, pdst, w, d, stype=None):
+        """``Network::connect(pSrc, pDst, w, d, type, size)``: synapse i joins
+        src i / dstSize to dst i % dstSize (src/net/Network.h:176-209)."""
+        s0, sn = self.pops[psrc]
+        d0, dn = self.pops[pdst]
+        i = np.arange(sn * dn, dtype=np.int64)
+        self.connect(s0 + i // dn, d0 + i % dn, w, d, stype)
+
+    def build(self, wdtype=np.float32) -> NetSpec:
+        kind = np.concatenate(self.kinds) if self.kinds else np.zeros(0, np.int8)
+        lif = np.concatenate(self.lifs) if self.lifs else np.zeros((0, LIF_NPARAM))
+        if any(a is not None for a in self.arr):
+            lens = np.array([0 if a is None else len(a) for a in self.arr], np.int64)
+            arr_off = np.concatenate([[0], np.cumsum(lens)]).astype(np.int64)
+            arr_t = (np.concatenate([a for a in self.arr if a is not None]).astype(np.int32)
+                     if lens.sum() else np.zeros(0, np.int32))
+        else:
+            arr_off, arr_t = None, None
+        cat = lambda xs, dt: (np.concatenate(xs).astype(dt) if xs else np.zeros(0, dt))  # noqa
+        return NetSpec(kind=kind, lif=lif, pre=cat(self.pre, np.int32),
+                       post=cat(self.post, np.int32), w=cat(self.w, wdtype),
+                       d=cat(self.d, wdtype), stype=cat(self.stype, np.int8),
+                       tau=cat(self.tau, wdtype), arr_off=arr_off, arr_t=arr_t,
+                       bern_p=np.concatenate(self.bern) if self.bern else None,
+                       pops=list(self.pops))
+
+
+# ---- the reference's own unit networks (test/gpu/lif_unit{1..4}_test.cpp) ----
+
+def lif_unit(k: int, wdtype=np.float32) -> NetSpec:
+    """3 populations x 10 LIF neurons, all-to-all p0->p1->p2, as in
+    test/gpu/lif_unit{1,2,3,4}_test.cpp (dt = 1e-4, run 0.1 s)."""
+    N = 10
+    refr = {4: (2e-4, 1e-4, 0.0)}.get(k, (0.0, 0.0, 0.0))
+    b = Builder()
+    b.population(np.tile(lif_row(0.0, 0.0, 0.0, 1.0e-1, 50.0e-3, refr[0], 1.0, 1.0, 15.0e-3,
+                                 100.0e-1, 1.0, 1.0), (N, 1)))
+    b.population(np.tile(lif_row(0.0, 0.0, 0.0, 1.0e-1, 50.0e-3, refr[1], 1.0, 1.0, 15.0e-3,
+                                 0.0e-3, 1.0, 1.0), (N, 1)))
+    b.population(np.tile(lif_row(0.0, 0.0, 0.0, 1.0e-1, 50.0e-3, refr[2], 1.0, 1.0, 15.0e-3,
+                                 0.0e-3, 1.0, 1.0), (N, 1)))
+    if k == 2:
+        i = np.arange(N * N, dtype=np.float64)
+        w0 = 1e-4 + i / float(N * N) * 2e-3
+        w1 = 2.1e-3 - i / float(N * N) * 2e-3
+    else:
+        w0 = np.full(N * N, 1e-5)
+        w1 = np.full(N * N, 2e-5)
+    d0 = np.full(N * N, 2e-4 if k == 3 else 1e-4)
+    d1 = np.full(N * N, 1e-4)
+    # the reference stores these into `real` arrays before connect()
+    cast = (lambda x: x.astype(np.float32)) if wdtype == np.float32 else (lambda x: x)
+    b.connect_all_to_all(0, 1, cast(w0), cast(d0))
+    b.connect_all_to_all(1, 2, cast(w1), cast(d1))
+    return b.build(wdtype)
+
+
+# ---- synthetic workloads -------------------------------------------------------
+
+DT = 1e-4
+
+
+def c0(seed: int = 1234, n: int = 1000, tau_syn_E=5e-3, tau_syn_I=10e-3) -> NetSpec:
+    """Plain method, small: 2 x n LIF (tau_m 20 ms, tau_refrac 2 ms, v_rest =
+    v_reset = 0, v_thresh 15 mV, R = 1), i_offset ~ U[14.5, 16.5), all-to-all
+    p0->p1 with w ~ U[0, 0.2) and integer delays U{1..6} steps."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    tau_m = 20e-3
+    b = Builder()
+    for _ in range(2):
+        ioff = rng.uniform(14.5, 16.5, n).astype(np.float32).astype(np.float64)
+        rows = np.tile(lif_row(0.0, 0.0, 0.0, tau_m, tau_m, 2e-3, tau_syn_E, tau_syn_I, 15.0,
+                               0.0), (n, 1))
+        rows[:, 9] = ioff
+        b.population(rows)
+    w = rng.uniform(0.0, 0.2, n * n).astype(np.float32)
+    k = rng.integers(1, 7, n * n)
+    d = k.astype(np.float32) * np.float32(DT)
+    b.connect_all_to_all(0, 1, w, d)
+    return b.build()
+
+
+def standard(npop=4, n=1000, fanout=100, w0=1.0, inh_frac=0.25, dmax=6, remote_frac=0.2,
+             drive_rate_hz=0.0, n_drive=None, drive_w=None, seed=1, tau_refrac=2e-3,
+             i_offset=(0.0, 0.0), tau_syn_E=5e-3, tau_syn_I=10e-3, v_thresh=15.0):
+    """Feed-forward chain of ``npop`` LIF populations of ``n`` neurons.
+
+    Population i projects to i+1; a ``remote_frac`` share of populations also
+    project to one uniformly chosen remote population (not i, not i+1).  Each
+    pre neuron has ``fanout`` distinct targets per projection, drawn by
+    stratified sampling (one target in each of ``fanout`` equal strata).
+    Excitatory weights ~ U[0, 0.7 w0), a share ``inh_frac`` inhibitory with
+    weights ~ U[-0.5 w0, 0); integer delays U{1..dmax} steps.  Population 0 is
+    driven by ``n_drive`` Bernoulli sources firing at ``drive_rate_hz`` (one
+    projection drive -> pop 0 with the same fan-out rule).  Synapses are
+    created source population by source population, pre-major, so every
+    target's inputs arrive in ascending (pre, delay) creation order.
+    """
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = Builder()
+    tau_m = 20e-3
+    drive = None
+    if drive_rate_hz > 0:
+        nd = n if n_drive is None else n_drive
+        p = min(1.0, drive_rate_hz * DT)
+        thr = np.uint32(min(0xFFFFFFFF, int(round(p * 4294967296.0))))
+        drive = b.population(np.zeros((nd, LIF_NPARAM)), kind=BERNOULLI, bern_p=thr)
+    pops = []
+    for _ in range(npop):
+        rows = np.tile(lif_row(0.0, 0.0, 0.0, tau_m, tau_m, tau_refrac, tau_syn_E, tau_syn_I,
+                               v_thresh, 0.0), (n, 1))
+        if i_offset[1] > i_offset[0]:
+            rows[:, 9] = rng.uniform(i_offset[0], i_offset[1], n).astype(np.float32)
+        pops.append(b.population(rows))
+    # projections: (src pop index in b, dst pop index in b)
+    proj = []
+    if drive is not None:
+        proj.append((drive, pops[0]))
+    remotes = {}
+    for i in range(npop):
+        if rng.random() < remote_frac and npop > 2:
+            choices = [j for j in range(npop) if j != i and j != i + 1]
+            remotes[i] = int(rng.choice(choices))
+    for i in range(npop):
+        if i + 1 < npop:
+            proj.append((pops[i], pops[i + 1]))
+        if i in remotes:
+            proj.append((pops[i], pops[remotes[i]]))
+    proj.sort(key=lambda x: x[0])  # creation: ascending source population
+    for src, dst in proj:
+        s0, sn = b.pops[src]
+        d0, dn = b.pops[dst]
+        f = min(fanout, dn)
+        strata = dn / f
+        pre = np.repeat(np.arange(s0, s0 + sn, dtype=np.int64), f)
+        base = (np.arange(f) * strata)[None, :]
+        off = np.floor(base + rng.random((sn, f)) * strata).astype(np.int64)
+        off = np.minimum(off, dn - 1).reshape(-1)
+        post = d0 + off
+        m = sn * f
+        if src == drive:
+            wv = (np.full(m, 0.7 * w0) if drive_w is None else np.full(m, drive_w))
+            st = np.zeros(m, np.int8)
+        else:
+            inh = rng.random(m) < inh_frac
+            wv = np.where(inh, rng.uniform(-0.5 * w0, 0.0, m), rng.uniform(0.0, 0.7 * w0, m))
+            st = inh.astype(np.int8)
+        k = rng.integers(1, dmax + 1, m)
+        b.connect(pre, post, wv.astype(np.float32), k.astype(np.float32) * np.float32(DT),
+                  st)
+    spec = b.build()
+    spec.seed = seed
+    return spec
+
+
+def random_net(seed, n_lif=40, n_arr=5, n_bern=3, p_conn=0.2, dmax=5, allow_dup=False,
+               shuffle=False, spike_prob=0.05, steps=200):
+    """Small random network for fuzzing: LIF + array + Bernoulli sources with
+    mixed-sign weights and random integer delays.  ``shuffle`` permutes the
+    creation order of synapses (the exact oracle must follow it)."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = Builder()
+    if n_arr:
+        sp = [np.sort(rng.choice(steps, size=rng.integers(0, steps // 4 + 1), replace=False))
+              for _ in range(n_arr)]
+        b.population(np.zeros((n_arr, LIF_NPARAM)), kind=ARRAY, spikes=sp)
+    if n_bern:
+        b.population(np.zeros((n_bern, LIF_NPARAM)), kind=BERNOULLI,
+                     bern_p=np.uint32(int(spike_prob * 2 ** 32)))
+    rows = np.tile(lif_row(0.0, -0.5, 0.0, 0.02, 0.02, 1e-3, 5e-3, 8e-3, 1.0, 0.0), (n_lif, 1))
+    rows[:, 9] = rng.uniform(0.5, 1.6, n_lif)
+    rows[:, 5] = rng.choice([0.0, 1e-4, 2e-4, 5e-4], n_lif)
+    b.population(rows)
+    n = b.n
+    pre, post = np.nonzero(rng.random((n, n)) < p_conn)
+    # no synapse may target a source neuron (they ignore input anyway)
+    keep = b_kind(b)[post] == LIF
+    pre, post = pre[keep], post[keep]
+    if allow_dup:
+        extra = rng.integers(0, len(pre), len(pre) // 5 + 1) if len(pre) else np.zeros(0, int)
+        pre = np.concatenate([pre, pre[extra]])
+        post = np.concatenate([post, post[extra]])
+    order = np.lexsort((post, pre))
+    pre, post = pre[order], post[order]
+    m = len(pre)
+    w = np.where(rng.random(m) < 0.3, -rng.uniform(0, 0.4, m), rng.uniform(0, 0.3, m))
+    k = rng.integers(1, dmax + 1, m)
+    if shuffle and m:
+        perm = rng.permutation(m)
+        pre, post, w, k = pre[perm], post[perm], w[perm], k[perm]
+    b.connect(pre, post, w.astype(np.float32), k.astype(np.float32) * np.float32(DT),
+              (w < 0).astype(np.int8))
+    spec = b.build()
+    spec.seed = seed * 7919 + 17
+    return spec
+
+
+def b_kind(b: Builder):
+    return np.concatenate(b.kinds)

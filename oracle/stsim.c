@@ -1,0 +1,149 @@
+/*
+ * stsim.c -- CPU oracle for the BSim simulation step (TEST INFRASTRUCTURE).
+ *
+ * Only tests/, __graft_entry__.smoke() and bench.py's cpu_baseline leg use
+ * this file; the product library never links or calls it.  Built with
+ * -ffp-contract=off and without -ffast-math so that every REAL operation is
+ * rounded exactly as written (the reference CPU build: g++ -O3, no FMA).
+ *
+ * Precision-generic body: stsim_body.inc.  Semantics: see bsim_oracle.h.
+ */
+#include <math.h>
+#include <stdint.h>
+#include <stdlib.h>
+#include <string.h>
+
+#include "bsim_oracle.h"
+
+/* ---- small utilities ----------------------------------------------------- */
+
+typedef struct bso_vec {
+    int64_t *v;
+    int64_t n, cap;
+} bso_vec;
+
+static int bso_vec_push(bso_vec *b, int64_t x)
+{
+    if (b->n == b->cap) {
+        int64_t nc = b->cap ? b->cap * 2 : 256;
+        int64_t *nv = (int64_t *)realloc(b->v, (size_t)nc * sizeof(int64_t));
+        if (!nv) return -1;
+        b->v = nv;
+        b->cap = nc;
+    }
+    b->v[b->n++] = x;
+    return 0;
+}
+
+static int cmp_i64(const void *a, const void *b)
+{
+    int64_t x = *(const int64_t *)a, y = *(const int64_t *)b;
+    return (x > y) - (x < y);
+}
+
+/* LSD radix sort of non-negative int64 keys, 16 bits per pass. */
+static void bso_sort_i64(int64_t *v, int64_t n)
+{
+    if (n < 2) return;
+    if (n < 4096) {
+        qsort(v, (size_t)n, sizeof(int64_t), cmp_i64);
+        return;
+    }
+    int64_t mx = 0;
+    for (int64_t i = 0; i < n; i++) if (v[i] > mx) mx = v[i];
+    int64_t *tmp = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+    if (!tmp) {
+        qsort(v, (size_t)n, sizeof(int64_t), cmp_i64);
+        return;
+    }
+    int64_t *src = v, *dst = tmp;
+    static int64_t cnt[65537];
+    for (int shift = 0; shift < 64 && (mx >> shift) != 0; shift += 16) {
+        memset(cnt, 0, sizeof(cnt));
+        for (int64_t i = 0; i < n; i++) cnt[((src[i] >> shift) & 0xFFFF) + 1]++;
+        for (int j = 0; j < 65536; j++) cnt[j + 1] += cnt[j];
+        for (int64_t i = 0; i < n; i++) dst[cnt[(src[i] >> shift) & 0xFFFF]++] = src[i];
+        int64_t *t = src; src = dst; dst = t;
+    }
+    if (src != v) memcpy(v, src, (size_t)n * sizeof(int64_t));
+    free(tmp);
+}
+
+/* Stable merge sort of synapse indices by (pre, delay_steps, index). */
+static int key3_less(int64_t a, int64_t b, const int32_t *pre, const int32_t *ds)
+{
+    if (pre[a] != pre[b]) return pre[a] < pre[b];
+    if (ds[a] != ds[b]) return ds[a] < ds[b];
+    return a < b;
+}
+
+static void bso_sort_by_key3(int64_t *v, int64_t n, const int32_t *pre, const int32_t *ds)
+{
+    if (n < 2) return;
+    int64_t *tmp = (int64_t *)malloc((size_t)n * sizeof(int64_t));
+    if (!tmp) return;
+    for (int64_t width = 1; width < n; width *= 2) {
+        for (int64_t lo = 0; lo < n; lo += 2 * width) {
+            int64_t mid = lo + width < n ? lo + width : n;
+            int64_t hi = lo + 2 * width < n ? lo + 2 * width : n;
+            int64_t i = lo, j = mid, o = lo;
+            while (i < mid && j < hi) tmp[o++] = key3_less(v[j], v[i], pre, ds) ? v[j++] : v[i++];
+            while (i < mid) tmp[o++] = v[i++];
+            while (j < hi) tmp[o++] = v[j++];
+        }
+        memcpy(v, tmp, (size_t)n * sizeof(int64_t));
+    }
+    free(tmp);
+}
+
+uint32_t bso_hash(uint64_t seed, uint32_t id, uint32_t t)
+{
+    uint64_t z = seed ^ (((uint64_t)id << 32) | (uint64_t)t);
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (uint32_t)(z >> 32);
+}
+
+int bso_csr_order_ok(int32_t n, int64_t s, const int32_t *pre, const int32_t *post,
+                     const int32_t *delay_steps)
+{
+    int64_t *last = (int64_t *)malloc((size_t)(n > 0 ? n : 1) * sizeof(int64_t));
+    if (!last) return 0;
+    for (int32_t i = 0; i < n; i++) last[i] = -1;
+    int ok = 1;
+    for (int64_t k = 0; k < s; k++) {
+        int64_t key = ((int64_t)pre[k] << 20) | (int64_t)(delay_steps[k] & 0xFFFFF);
+        if (key < last[post[k]]) { ok = 0; break; }
+        last[post[k]] = key;
+    }
+    free(last);
+    return ok;
+}
+
+/* ---- precision instantiations -------------------------------------------- */
+
+#define REAL float
+#define SUF f32
+#define EXP_R expf
+#define FABS_R fabsf
+#define ROUND_R roundf
+#include "stsim_body.inc"
+#undef REAL
+#undef SUF
+#undef EXP_R
+#undef FABS_R
+#undef ROUND_R
+
+#define REAL double
+#define SUF f64
+#define EXP_R exp
+#define FABS_R fabs
+#define ROUND_R round
+#include "stsim_body.inc"
+#undef REAL
+#undef SUF
+#undef EXP_R
+#undef FABS_R
+#undef ROUND_R
