@@ -1,0 +1,531 @@
+/*
+ * capi.cpp -- the C ABI declared in include/bsim_amd.h: host network store,
+ * device upload, the per-cycle launch sequence and read-back.
+ */
+#include <hip/hip_runtime.h>
+
+#include <cstdio>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/bsim_amd.h"
+#include "layout.h"
+#include "network.h"
+
+extern "C" hipError_t bsim_launch_step(const bsim_dev *d, int32_t t, int32_t flags,
+                                       hipStream_t stream);
+
+static thread_local std::string g_err;
+
+static int fail(int code, const std::string &msg)
+{
+    g_err = msg;
+    return code;
+}
+
+struct bsim_network {
+    bsim::HostNet net;
+};
+
+struct bsim_sim {
+    int device = 0;
+    hipStream_t stream = nullptr;
+    bsim_dev d{};
+    std::vector<void *> allocs;
+    int64_t device_bytes = 0;
+    int32_t cycle = 0;
+    int32_t finished_at = -1;
+    int64_t *ev_fin = nullptr;   /* deliveries of the deliver-only launch */
+    int32_t rec_steps = 0, n_rec = 0;
+    int64_t n_src = 0, n_off = 0, s = 0;
+};
+
+extern "C" {
+
+int32_t bsim_abi_version(void) { return BSIM_ABI_VERSION; }
+
+const char *bsim_last_error(void) { return g_err.c_str(); }
+
+/* ---- network construction ------------------------------------------------ */
+
+bsim_network *bsim_net_create(void)
+{
+    try {
+        return new bsim_network();
+    } catch (...) {
+        g_err = "out of host memory";
+        return nullptr;
+    }
+}
+
+void bsim_net_destroy(bsim_network *net) { delete net; }
+
+int32_t bsim_net_add_lif(bsim_network *net, int32_t n, const float *params)
+{
+    if (!net || n < 0 || (n > 0 && !params)) return fail(BSIM_E_ARG, "bsim_net_add_lif: bad argument");
+    auto &h = net->net;
+    if ((int64_t)h.n() + n > 0x7FFFFFFF) return fail(BSIM_E_ARG, "too many neurons");
+    const int32_t first = h.n();
+    try {
+        h.kind.insert(h.kind.end(), (size_t)n, (int8_t)BSIM_KIND_LIF);
+        h.lif.insert(h.lif.end(), params, params + (size_t)n * bsim::kLifParams);
+        h.arr_off.insert(h.arr_off.end(), (size_t)n, h.arr_off.back());
+        h.bern_p.insert(h.bern_p.end(), (size_t)n, 0u);
+    } catch (...) {
+        return fail(BSIM_E_NOMEM, "out of host memory");
+    }
+    return first;
+}
+
+int32_t bsim_net_add_array(bsim_network *net, int32_t n, const int64_t *offsets,
+                           const int32_t *cycles, const float *taus)
+{
+    if (!net || n < 0 || (n > 0 && !offsets) || (offsets && offsets[0] != 0))
+        return fail(BSIM_E_ARG, "bsim_net_add_array: bad argument");
+    auto &h = net->net;
+    const int32_t first = h.n();
+    try {
+        for (int32_t i = 0; i < n; i++) {
+            if (offsets[i + 1] < offsets[i]) return fail(BSIM_E_ARG, "offsets must not decrease");
+            for (int64_t j = offsets[i] + 1; j < offsets[i + 1]; j++)
+                if (cycles[j] < cycles[j - 1]) return fail(BSIM_E_ARG, "spike cycles must ascend");
+        }
+        h.kind.insert(h.kind.end(), (size_t)n, (int8_t)BSIM_KIND_ARRAY);
+        for (int32_t i = 0; i < n; i++) {
+            float row[bsim::kLifParams] = {0};
+            if (taus) {
+                row[10] = taus[2 * i];
+                row[11] = taus[2 * i + 1];
+            } else {
+                row[10] = 1e-3f;   // CompositeNeuron's default taus (CompositeNeuron.h:160)
+                row[11] = 1e-3f;
+            }
+            h.lif.insert(h.lif.end(), row, row + bsim::kLifParams);
+            const int64_t base = h.arr_off.back();
+            h.arr_t.insert(h.arr_t.end(), cycles + offsets[i], cycles + offsets[i + 1]);
+            h.arr_off.push_back(base + offsets[i + 1] - offsets[i]);
+        }
+        h.bern_p.insert(h.bern_p.end(), (size_t)n, 0u);
+    } catch (...) {
+        return fail(BSIM_E_NOMEM, "out of host memory");
+    }
+    return first;
+}
+
+int32_t bsim_net_add_bernoulli(bsim_network *net, int32_t n, const uint32_t *p, uint64_t seed)
+{
+    if (!net || n < 0 || (n > 0 && !p)) return fail(BSIM_E_ARG, "bsim_net_add_bernoulli: bad argument");
+    auto &h = net->net;
+    const int32_t first = h.n();
+    try {
+        h.kind.insert(h.kind.end(), (size_t)n, (int8_t)BSIM_KIND_BERNOULLI);
+        for (int32_t i = 0; i < n; i++) {
+            float row[bsim::kLifParams] = {0};
+            row[10] = 1e-3f;
+            row[11] = 1e-3f;
+            h.lif.insert(h.lif.end(), row, row + bsim::kLifParams);
+        }
+        h.arr_off.insert(h.arr_off.end(), (size_t)n, h.arr_off.back());
+        h.bern_p.insert(h.bern_p.end(), p, p + n);
+        h.seed = seed;
+    } catch (...) {
+        return fail(BSIM_E_NOMEM, "out of host memory");
+    }
+    return first;
+}
+
+int64_t bsim_net_connect(bsim_network *net, int64_t count, const int32_t *pre,
+                         const int32_t *post, const float *w, const float *d,
+                         const int8_t *stype, const float *tau)
+{
+    if (!net || count < 0 || (count > 0 && (!pre || !post || !w || !d)))
+        return fail(BSIM_E_ARG, "bsim_net_connect: bad argument");
+    auto &h = net->net;
+    try {
+        h.pre.insert(h.pre.end(), pre, pre + count);
+        h.post.insert(h.post.end(), post, post + count);
+        h.w.insert(h.w.end(), w, w + count);
+        h.d.insert(h.d.end(), d, d + count);
+        if (stype)
+            h.stype.insert(h.stype.end(), stype, stype + count);
+        else
+            h.stype.insert(h.stype.end(), (size_t)count, (int8_t)0);
+        if (tau) {
+            if (h.tau.size() < h.pre.size() - (size_t)count) h.tau.resize(h.pre.size() - (size_t)count, 0.0f);
+            h.tau.insert(h.tau.end(), tau, tau + count);
+        } else if (!h.tau.empty()) {
+            h.tau.insert(h.tau.end(), (size_t)count, 0.0f);
+        }
+    } catch (...) {
+        return fail(BSIM_E_NOMEM, "out of host memory");
+    }
+    return h.s();
+}
+
+int32_t bsim_net_num_neurons(const bsim_network *net) { return net ? net->net.n() : BSIM_E_ARG; }
+
+int64_t bsim_net_num_synapses(const bsim_network *net) { return net ? net->net.s() : BSIM_E_ARG; }
+
+int32_t bsim_net_lif_constants(const bsim_network *net, float dt, float *out9, int32_t *refrac_time)
+{
+    if (!net || !out9 || !refrac_time) return fail(BSIM_E_ARG, "bad argument");
+    const auto &h = net->net;
+    for (int32_t i = 0; i < h.n(); i++) {
+        if (h.kind[i] == BSIM_KIND_LIF) {
+            bsim::lif_constants(&h.lif[(size_t)i * bsim::kLifParams], dt, out9 + (size_t)i * 9,
+                                &refrac_time[i]);
+        } else {
+            memset(out9 + (size_t)i * 9, 0, 9 * sizeof(float));
+            refrac_time[i] = 0;
+        }
+    }
+    return 0;
+}
+
+int32_t bsim_net_layout(const bsim_network *net, float dt, int32_t tile, int32_t *ntiles,
+                        int64_t *s, int32_t *dmax, int64_t *tile_syn, uint16_t *syn_tgt,
+                        float *syn_w)
+{
+    if (!net) return fail(BSIM_E_ARG, "bad argument");
+    bsim::Layout L;
+    std::string err;
+    try {
+        if (bsim::build_layout(net->net, dt, tile, L, err) != 0) return fail(BSIM_E_NETWORK, err);
+    } catch (...) {
+        return fail(BSIM_E_NOMEM, "out of host memory");
+    }
+    if (ntiles) *ntiles = L.ntiles;
+    if (s) *s = L.s;
+    if (dmax) *dmax = L.dmax;
+    if (tile_syn) memcpy(tile_syn, L.tile_syn.data(), L.tile_syn.size() * sizeof(int64_t));
+    if (syn_tgt) memcpy(syn_tgt, L.syn_tgt.data(), L.syn_tgt.size() * sizeof(uint16_t));
+    if (syn_w) memcpy(syn_w, L.syn_w.data(), L.syn_w.size() * sizeof(float));
+    return 0;
+}
+
+/* ---- device simulation ---------------------------------------------------- */
+
+}  // extern "C"
+
+static void sim_free(bsim_sim *sim)
+{
+    if (!sim) return;
+    hipSetDevice(sim->device);
+    if (sim->stream) hipStreamSynchronize(sim->stream);
+    for (void *p : sim->allocs) hipFree(p);
+    if (sim->stream) hipStreamDestroy(sim->stream);
+    delete sim;
+}
+
+template <class T>
+static T *dev_alloc(bsim_sim *sim, size_t count, hipError_t &e)
+{
+    void *p = nullptr;
+    const size_t bytes = (count ? count : 1) * sizeof(T);
+    e = hipMalloc(&p, bytes);
+    if (e != hipSuccess) return nullptr;
+    sim->allocs.push_back(p);
+    sim->device_bytes += (int64_t)bytes;
+    return (T *)p;
+}
+
+template <class T>
+static T *dev_upload(bsim_sim *sim, const std::vector<T> &v, hipError_t &e)
+{
+    T *p = dev_alloc<T>(sim, v.size(), e);
+    if (!p) return nullptr;
+    if (!v.empty()) e = hipMemcpyAsync(p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, sim->stream);
+    if (e != hipSuccess) return nullptr;
+    e = hipStreamSynchronize(sim->stream);
+    return e == hipSuccess ? p : nullptr;
+}
+
+extern "C" {
+
+bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_config *cfg)
+{
+    if (!net || !(dt > 0)) {
+        fail(BSIM_E_ARG, "bsim_sim_create: bad argument");
+        return nullptr;
+    }
+    bsim_sim_config c{};
+    if (cfg) c = *cfg;
+    if (c.rec_steps < 0 || c.n_rec < 0 || (c.n_rec > 0 && !c.rec_ids)) {
+        fail(BSIM_E_ARG, "bsim_sim_create: bad recording configuration");
+        return nullptr;
+    }
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
+        fail(BSIM_E_HIP, "no HIP device available");
+        return nullptr;
+    }
+    if (c.device < 0 || c.device >= ndev) {
+        fail(BSIM_E_ARG, "device ordinal out of range");
+        return nullptr;
+    }
+    hipError_t e = hipSetDevice(c.device);
+    if (e != hipSuccess) {
+        fail(BSIM_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+        return nullptr;
+    }
+    int cus = 256;
+    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device);
+    const int32_t N = net->net.n();
+    const int32_t tile = c.tile > 0 ? c.tile : bsim::default_tile(N, cus);
+    for (int32_t r = 0; r < c.n_rec; r++) {
+        if (c.rec_ids[r] < 0 || c.rec_ids[r] >= N) {
+            fail(BSIM_E_ARG, "recorded neuron id out of range");
+            return nullptr;
+        }
+    }
+    bsim::Layout L;
+    std::string err;
+    try {
+        if (bsim::build_layout(net->net, dt, tile, L, err) != 0) {
+            fail(BSIM_E_NETWORK, err);
+            return nullptr;
+        }
+    } catch (const std::bad_alloc &) {
+        fail(BSIM_E_NOMEM, "out of host memory while building the layout");
+        return nullptr;
+    }
+    bsim_sim *sim = new (std::nothrow) bsim_sim();
+    if (!sim) {
+        fail(BSIM_E_NOMEM, "out of host memory");
+        return nullptr;
+    }
+    sim->device = c.device;
+    e = hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking);
+    if (e != hipSuccess) {
+        fail(BSIM_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
+        delete sim;
+        return nullptr;
+    }
+    bsim_dev &d = sim->d;
+    d.n = L.n;
+    d.tile = L.tile;
+    d.ntiles = L.ntiles;
+    d.dmax = L.dmax;
+    d.seed = L.seed;
+    sim->s = L.s;
+    sim->n_src = (int64_t)L.src.size();
+    sim->n_off = (int64_t)L.off_tab.size();
+#define UP(field, vec)                                   \
+    do {                                                 \
+        d.field = dev_upload(sim, vec, e);               \
+        if (e != hipSuccess) goto hip_fail;              \
+    } while (0)
+    UP(kind, L.kind);
+    UP(Cm, L.Cm); UP(CE, L.CE); UP(CI, L.CI); UP(C_E, L.C_E); UP(C_I, L.C_I);
+    UP(v_tmp, L.v_tmp); UP(v_thresh, L.v_thresh); UP(v_reset, L.v_reset);
+    UP(refrac_time, L.refrac_time);
+    UP(arr_end, L.arr_end);
+    UP(arr_t, L.arr_t);
+    UP(bern_p, L.bern_p);
+    UP(tile_syn, L.tile_syn);
+    UP(syn_tgt, L.syn_tgt);
+    UP(syn_w, L.syn_w);
+    UP(tile_src, L.tile_src);
+    UP(src, L.src);
+    UP(off_tab, L.off_tab);
+    /* state */
+    UP(v, L.v_init);
+    UP(arr_cur, L.arr_cur0);
+#undef UP
+    {
+        const size_t n = (size_t)(L.n > 0 ? L.n : 1);
+        const size_t nt = (size_t)(L.ntiles > 0 ? L.ntiles : 1);
+        if (!(d.ie = dev_alloc<float>(sim, n, e))) goto hip_fail;
+        if (!(d.ii = dev_alloc<float>(sim, n, e))) goto hip_fail;
+        if (!(d.refrac = dev_alloc<int32_t>(sim, n, e))) goto hip_fail;
+        if (!(d.hist = dev_alloc<uint64_t>(sim, n, e))) goto hip_fail;
+        if (!(d.fire_count = dev_alloc<int32_t>(sim, n, e))) goto hip_fail;
+        if (!(d.act_cnt = dev_alloc<int32_t>(sim, 2 * nt, e))) goto hip_fail;
+        if (!(d.act_ids = dev_alloc<int32_t>(sim, 2 * n, e))) goto hip_fail;
+        if (!(d.act_hist = dev_alloc<uint64_t>(sim, 2 * n, e))) goto hip_fail;
+        if (!(d.ev_count = dev_alloc<int64_t>(sim, nt, e))) goto hip_fail;
+        if (!(sim->ev_fin = dev_alloc<int64_t>(sim, nt, e))) goto hip_fail;
+        hipMemsetAsync(d.ie, 0, n * 4, sim->stream);
+        hipMemsetAsync(d.ii, 0, n * 4, sim->stream);
+        hipMemsetAsync(d.refrac, 0, n * 4, sim->stream);
+        hipMemsetAsync(d.hist, 0, n * 8, sim->stream);
+        hipMemsetAsync(d.fire_count, 0, n * 4, sim->stream);
+        hipMemsetAsync(d.act_cnt, 0, 2 * nt * 4, sim->stream);
+        hipMemsetAsync(d.ev_count, 0, nt * 8, sim->stream);
+        hipMemsetAsync(sim->ev_fin, 0, nt * 8, sim->stream);
+        d.nwords = (L.n + 63) / 64;
+        if (c.rec_steps > 0) {
+            d.rec_steps = c.rec_steps;
+            d.rec_t0 = 0;
+            const size_t rb = (size_t)c.rec_steps * (size_t)(d.nwords > 0 ? d.nwords : 1);
+            if (!(d.raster = dev_alloc<uint64_t>(sim, rb, e))) goto hip_fail;
+            hipMemsetAsync(d.raster, 0, rb * 8, sim->stream);
+            if (c.n_rec > 0) {
+                std::vector<int32_t> idx(L.n, -1);
+                for (int32_t r = 0; r < c.n_rec; r++) idx[c.rec_ids[r]] = r;
+                d.rec_index = dev_upload(sim, idx, e);
+                if (e != hipSuccess) goto hip_fail;
+                d.n_rec = c.n_rec;
+                const size_t vb = (size_t)c.rec_steps * (size_t)c.n_rec;
+                if (!(d.rec_v = dev_alloc<float>(sim, vb, e))) goto hip_fail;
+                hipMemsetAsync(d.rec_v, 0, vb * 4, sim->stream);
+            }
+        }
+        sim->rec_steps = c.rec_steps;
+        sim->n_rec = d.n_rec;
+        e = hipStreamSynchronize(sim->stream);
+        if (e != hipSuccess) goto hip_fail;
+    }
+    return sim;
+hip_fail:
+    fail(e == hipErrorOutOfMemory ? BSIM_E_NOMEM : BSIM_E_HIP,
+         std::string("device setup failed: ") + hipGetErrorString(e));
+    sim_free(sim);
+    return nullptr;
+}
+
+int32_t bsim_sim_step(bsim_sim *sim, int32_t n)
+{
+    if (!sim || n < 0) return fail(BSIM_E_ARG, "bsim_sim_step: bad argument");
+    hipSetDevice(sim->device);
+    for (int32_t i = 0; i < n; i++) {
+        const int32_t t = sim->cycle;
+        const int32_t flags = (t > 0 ? 1 : 0) | 2;
+        hipError_t e = bsim_launch_step(&sim->d, t, flags, sim->stream);
+        if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("step launch: ") + hipGetErrorString(e));
+        sim->cycle++;
+    }
+    return 0;
+}
+
+int32_t bsim_sim_finish(bsim_sim *sim)
+{
+    if (!sim) return fail(BSIM_E_ARG, "bad argument");
+    if (sim->cycle == 0 || sim->finished_at == sim->cycle) return 0;
+    hipSetDevice(sim->device);
+    bsim_dev d = sim->d;
+    d.ev_count = sim->ev_fin;
+    hipMemsetAsync(sim->ev_fin, 0, (size_t)(d.ntiles > 0 ? d.ntiles : 1) * 8, sim->stream);
+    hipError_t e = bsim_launch_step(&d, sim->cycle, 1, sim->stream);
+    if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("finish launch: ") + hipGetErrorString(e));
+    sim->finished_at = sim->cycle;
+    return 0;
+}
+
+int32_t bsim_sim_sync(bsim_sim *sim)
+{
+    if (!sim) return fail(BSIM_E_ARG, "bad argument");
+    hipSetDevice(sim->device);
+    hipError_t e = hipStreamSynchronize(sim->stream);
+    if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("stream: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int32_t bsim_sim_cycle(const bsim_sim *sim) { return sim ? sim->cycle : BSIM_E_ARG; }
+
+}  // extern "C"
+
+template <class T>
+static hipError_t fetch(bsim_sim *sim, T *dst, const T *src, size_t count)
+{
+    if (!dst || !count) return hipSuccess;
+    hipError_t e = hipMemcpyAsync(dst, src, count * sizeof(T), hipMemcpyDeviceToHost, sim->stream);
+    if (e != hipSuccess) return e;
+    return hipStreamSynchronize(sim->stream);
+}
+
+extern "C" {
+
+int64_t bsim_sim_read_raster(bsim_sim *sim, int32_t *counts, int32_t *ids, int64_t cap,
+                             int64_t *needed)
+{
+    if (!sim) return fail(BSIM_E_ARG, "bad argument");
+    if (!sim->d.raster) return fail(BSIM_E_STATE, "raster recording was not enabled");
+    hipSetDevice(sim->device);
+    const int32_t steps = sim->cycle < sim->rec_steps ? sim->cycle : sim->rec_steps;
+    std::vector<uint64_t> bits((size_t)steps * sim->d.nwords);
+    hipError_t e = fetch(sim, bits.data(), sim->d.raster, bits.size());
+    if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("raster copy: ") + hipGetErrorString(e));
+    int64_t total = 0;
+    for (uint64_t b : bits) total += __builtin_popcountll(b);
+    if (needed) *needed = total;
+    if (total > cap || !counts || (total > 0 && !ids)) return fail(BSIM_E_CAPACITY, "raster buffer too small");
+    int64_t o = 0;
+    for (int32_t t = 0; t < steps; t++) {
+        int32_t c = 0;
+        for (int32_t wd = 0; wd < sim->d.nwords; wd++) {
+            uint64_t b = bits[(size_t)t * sim->d.nwords + wd];
+            while (b) {
+                const int bit = __builtin_ctzll(b);
+                b &= b - 1;
+                ids[o++] = wd * 64 + bit;
+                c++;
+            }
+        }
+        counts[t] = c;
+    }
+    return o;
+}
+
+int32_t bsim_sim_read_trace(bsim_sim *sim, float *v)
+{
+    if (!sim || !v) return fail(BSIM_E_ARG, "bad argument");
+    if (!sim->d.rec_v) return fail(BSIM_E_STATE, "no neuron is traced");
+    hipSetDevice(sim->device);
+    const int32_t steps = sim->cycle < sim->rec_steps ? sim->cycle : sim->rec_steps;
+    hipError_t e = fetch(sim, v, sim->d.rec_v, (size_t)steps * sim->n_rec);
+    if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("trace copy: ") + hipGetErrorString(e));
+    return steps;
+}
+
+int32_t bsim_sim_read_state(bsim_sim *sim, float *v, float *ie, float *ii, int32_t *refrac,
+                            int32_t *fire_count)
+{
+    if (!sim) return fail(BSIM_E_ARG, "bad argument");
+    hipSetDevice(sim->device);
+    const size_t n = (size_t)sim->d.n;
+    hipError_t e;
+    if ((e = fetch(sim, v, sim->d.v, n)) != hipSuccess ||
+        (e = fetch(sim, ie, sim->d.ie, n)) != hipSuccess ||
+        (e = fetch(sim, ii, sim->d.ii, n)) != hipSuccess ||
+        (e = fetch(sim, refrac, sim->d.refrac, n)) != hipSuccess ||
+        (e = fetch(sim, fire_count, sim->d.fire_count, n)) != hipSuccess)
+        return fail(BSIM_E_HIP, std::string("state copy: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int64_t bsim_sim_events(bsim_sim *sim)
+{
+    if (!sim) return fail(BSIM_E_ARG, "bad argument");
+    hipSetDevice(sim->device);
+    const size_t nt = (size_t)sim->d.ntiles;
+    std::vector<int64_t> a(nt), b(nt);
+    hipError_t e = fetch(sim, a.data(), sim->d.ev_count, nt);
+    if (e == hipSuccess && sim->finished_at == sim->cycle) e = fetch(sim, b.data(), sim->ev_fin, nt);
+    if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("event count copy: ") + hipGetErrorString(e));
+    int64_t s = 0;
+    for (size_t i = 0; i < nt; i++) s += a[i] + b[i];
+    return s;
+}
+
+int32_t bsim_sim_get_info(const bsim_sim *sim, bsim_sim_info *info)
+{
+    if (!sim || !info) return fail(BSIM_E_ARG, "bad argument");
+    info->n = sim->d.n;
+    info->tile = sim->d.tile;
+    info->ntiles = sim->d.ntiles;
+    info->dmax = sim->d.dmax;
+    info->s = sim->s;
+    info->device_bytes = sim->device_bytes;
+    info->n_src_entries = sim->n_src;
+    info->off_entries = sim->n_off;
+    return 0;
+}
+
+void *bsim_sim_stream(bsim_sim *sim) { return sim ? (void *)sim->stream : nullptr; }
+
+void bsim_sim_destroy(bsim_sim *sim) { sim_free(sim); }
+
+}  // extern "C"

@@ -1,0 +1,277 @@
+/*
+ * network.cpp -- LIF/synapse reset and the tile-major (pre, delay) CSR build.
+ * Compiled with -ffp-contract=off and without fast-math: the constants are
+ * computed in float in the reference's operation order, so they carry the
+ * reference's bits (expf from the host libm, as libstdc++'s exp(float)).
+ */
+#include "network.h"
+
+#include <math.h>
+
+#include <algorithm>
+#include <cstring>
+#include <numeric>
+
+namespace bsim {
+
+static const float kZero = 1e-10f;  // `const real ZERO = 1e-10;` (src/utils/constant.h:19)
+
+void lif_constants(const float *p, float dt, float out[9], int32_t *refrac_time)
+{
+    const float v_init = p[0], v_rest = p[1], v_reset = p[2], cm = p[3];
+    const float tau_m = p[4], tau_refrac = p[5], tau_E = p[6], tau_I = p[7];
+    const float v_thresh = p[8], i_offset = p[9];
+    float rm = 1.0f;
+    if (fabsf(cm) > kZero) {
+        rm = tau_m / cm;
+    }
+    const float Cm = tau_m > 0 ? expf(-dt / tau_m) : 0.0f;
+    const float CE = tau_E > 0 ? expf(-dt / tau_E) : 0.0f;
+    const float CI = tau_I > 0 ? expf(-dt / tau_I) : 0.0f;
+    float v_tmp = i_offset * rm + v_rest;
+    v_tmp *= (1 - Cm);
+    float C_E = rm * tau_E / (tau_E - tau_m);
+    float C_I = rm * tau_I / (tau_I - tau_m);
+    C_E = C_E * (CE - Cm);
+    C_I = C_I * (CI - Cm);
+    out[0] = Cm;
+    out[1] = CE;
+    out[2] = CI;
+    out[3] = C_E;
+    out[4] = C_I;
+    out[5] = v_tmp;
+    out[6] = v_thresh;
+    out[7] = v_reset;
+    out[8] = v_init;
+    *refrac_time = (int32_t)(tau_refrac / dt);
+}
+
+bool synapse_reset(float w, float delay, float tau, float dt, float *w_out, int32_t *ds)
+{
+    const int32_t steps = (int32_t)roundf(delay / dt);
+    const float resid = delay - dt * steps;
+    if (!(fabsf(tau) > kZero || fabsf(resid) < kZero)) {
+        return false;
+    }
+    if (fabsf(tau) > kZero && fabsf(resid) > kZero) {
+        const float c1 = expf(-(resid) / tau);
+        w = w * c1;
+    }
+    *w_out = w;
+    *ds = steps;
+    return true;
+}
+
+int32_t default_tile(int32_t n, int32_t cus)
+{
+    if (cus <= 0) cus = 256;
+    int64_t want = ((int64_t)n + (int64_t)cus * 4 - 1) / ((int64_t)cus * 4);
+    int64_t t = ((want + 63) / 64) * 64;
+    if (t < 64) t = 64;
+    if (t > 2048) t = 2048;
+    return (int32_t)t;
+}
+
+int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::string &err)
+{
+    const int32_t N = net.n();
+    const int64_t S = net.s();
+    if (tile <= 0 || tile % 64 != 0 || tile > BSIM_MAX_TILE) {
+        err = "tile size must be a positive multiple of 64 up to " + std::to_string(BSIM_MAX_TILE);
+        return -1;
+    }
+    if (tile > 65536) {
+        err = "tile size exceeds the 16-bit local target index";
+        return -1;
+    }
+    L = Layout();
+    L.n = N;
+    L.s = S;
+    L.tile = tile;
+    L.ntiles = N > 0 ? (N + tile - 1) / tile : 0;
+    L.seed = net.seed;
+    L.kind = net.kind;
+    L.Cm.assign(N, 0.f); L.CE.assign(N, 0.f); L.CI.assign(N, 0.f);
+    L.C_E.assign(N, 0.f); L.C_I.assign(N, 0.f); L.v_tmp.assign(N, 0.f);
+    L.v_thresh.assign(N, 0.f); L.v_reset.assign(N, 0.f); L.v_init.assign(N, 0.f);
+    L.refrac_time.assign(N, 0);
+    L.arr_cur0.assign(N, 0);
+    L.arr_end.assign(N, 0);
+    L.bern_p.assign(N, 0u);
+    for (int32_t i = 0; i < N; i++) {
+        if (net.kind[i] == BSIM_KIND_LIF) {
+            float c[9];
+            lif_constants(&net.lif[(size_t)i * kLifParams], dt, c, &L.refrac_time[i]);
+            L.Cm[i] = c[0]; L.CE[i] = c[1]; L.CI[i] = c[2]; L.C_E[i] = c[3]; L.C_I[i] = c[4];
+            L.v_tmp[i] = c[5]; L.v_thresh[i] = c[6]; L.v_reset[i] = c[7]; L.v_init[i] = c[8];
+        } else if (net.kind[i] == BSIM_KIND_ARRAY) {
+            L.arr_cur0[i] = (int32_t)net.arr_off[i];
+            L.arr_end[i] = (int32_t)net.arr_off[i + 1];
+        } else if (net.kind[i] == BSIM_KIND_BERNOULLI) {
+            L.bern_p[i] = net.bern_p[i];
+        } else {
+            err = "unknown neuron kind";
+            return -1;
+        }
+    }
+    if (net.arr_t.size() > 0x7FFFFFFF) {
+        err = "too many array-source spike times";
+        return -1;
+    }
+    L.arr_t = net.arr_t;
+
+    /* synapse reset: weight and delay steps */
+    std::vector<float> w(S);
+    std::vector<uint8_t> ds(S);
+    int32_t dmax = 1;
+    for (int64_t k = 0; k < S; k++) {
+        const int32_t a = net.pre[k], b = net.post[k];
+        if (a < 0 || a >= N || b < 0 || b >= N) {
+            err = "synapse " + std::to_string(k) + " has an endpoint outside the network";
+            return -2;
+        }
+        float tau_in = net.tau.empty() ? 0.0f : net.tau[k];
+        float tau;
+        if (fabsf(tau_in) > kZero) {
+            tau = tau_in;
+        } else {
+            const float *pd = &net.lif[(size_t)b * kLifParams];
+            tau = net.stype[k] ? pd[11] : pd[10];
+        }
+        int32_t steps;
+        if (!synapse_reset(net.w[k], net.d[k], tau, dt, &w[k], &steps)) {
+            err = "synapse " + std::to_string(k) +
+                  ": delay is not a whole number of steps and its tau is zero";
+            return -3;
+        }
+        if (steps < 1 || steps > BSIM_MAX_DELAY) {
+            err = "synapse " + std::to_string(k) + ": delay of " + std::to_string(steps) +
+                  " steps is outside 1.." + std::to_string(BSIM_MAX_DELAY);
+            return -4;
+        }
+        ds[k] = (uint8_t)steps;
+        if (steps > dmax) dmax = steps;
+    }
+    L.dmax = dmax;
+
+    /* Ordering property: the GPU sums a target's inputs in (pre, delay,
+     * creation) order; the reference sums them in creation order
+     * (Network.cpp:324-327).  They agree when each target's incoming
+     * synapses, taken in creation order, have non-decreasing (pre, delay). */
+    {
+        std::vector<int64_t> last(N, -1);
+        for (int64_t k = 0; k < S; k++) {
+            const int64_t key = ((int64_t)net.pre[k] << 8) | ds[k];
+            int64_t &l = last[net.post[k]];
+            if (key < l) {
+                err = "synapse " + std::to_string(k) + " (" + std::to_string(net.pre[k]) + " -> " +
+                      std::to_string(net.post[k]) +
+                      ") was created after a synapse into the same target from a later "
+                      "(pre, delay); the GPU step requires each target's inputs to be created "
+                      "in ascending (pre, delay) order";
+                return -5;
+            }
+            l = key;
+        }
+    }
+
+    /* bucket synapses by target tile, keeping creation order */
+    const int32_t NT = L.ntiles;
+    L.tile_syn.assign((size_t)NT + 1, 0);
+    for (int64_t k = 0; k < S; k++) L.tile_syn[net.post[k] / tile + 1]++;
+    for (int32_t t = 0; t < NT; t++) L.tile_syn[t + 1] += L.tile_syn[t];
+    std::vector<int64_t> order(S);
+    {
+        std::vector<int64_t> fill(L.tile_syn.begin(), L.tile_syn.end() - 1);
+        for (int64_t k = 0; k < S; k++) order[fill[net.post[k] / tile]++] = k;
+    }
+    for (int32_t t = 0; t < NT; t++) {
+        if (L.tile_syn[t + 1] - L.tile_syn[t] >= ((int64_t)1 << 26)) {
+            err = "a tile receives 2^26 or more synapses; use a smaller tile";
+            return -6;
+        }
+    }
+    L.syn_tgt.resize(S);
+    L.syn_w.resize(S);
+    L.tile_src.assign((size_t)NT + 1, 0);
+    std::vector<std::vector<bsim_src_entry>> tsrc(NT);
+    std::vector<std::vector<uint32_t>> toff(NT);
+    const int32_t D = dmax;
+    int fail = 0;
+
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int32_t t = 0; t < NT; t++) {
+        const int64_t lo = L.tile_syn[t], hi = L.tile_syn[t + 1], m = hi - lo;
+        /* key: pre (31 bits) | delay (6 bits) | rank within tile (26 bits) */
+        std::vector<uint64_t> key(m);
+        for (int64_t e = 0; e < m; e++) {
+            const int64_t k = order[lo + e];
+            key[e] = ((uint64_t)(uint32_t)net.pre[k] << 32) | ((uint64_t)ds[k] << 26) | (uint64_t)e;
+        }
+        std::sort(key.begin(), key.end());
+        for (int64_t e = 0; e < m; e++) {
+            const int64_t k = order[lo + (int64_t)(key[e] & ((1u << 26) - 1))];
+            L.syn_tgt[lo + e] = (uint16_t)(net.post[k] - t * tile);
+            L.syn_w[lo + e] = w[k];
+        }
+        /* source entries: one per source tile present, pre range clipped */
+        int64_t e = 0;
+        while (e < m) {
+            const int32_t p0 = (int32_t)(key[e] >> 32);
+            const int32_t st = p0 / tile;
+            int64_t f = e;
+            int32_t plast = p0;
+            while (f < m && (int32_t)(key[f] >> 32) / tile == st) {
+                plast = (int32_t)(key[f] >> 32);
+                f++;
+            }
+            bsim_src_entry se;
+            se.src_tile = st;
+            se.lo = p0;
+            se.hi = plast + 1;
+            se.pad = 0;
+            se.off_base = (int64_t)toff[t].size();
+            const int64_t cnt = (int64_t)(se.hi - se.lo) * D + 1;
+            const size_t base = toff[t].size();
+            toff[t].resize(base + cnt);
+            int64_t pos = e;
+            for (int32_t p = se.lo; p < se.hi; p++) {
+                for (int32_t d = 1; d <= D; d++) {
+                    toff[t][base + (size_t)(p - se.lo) * D + (d - 1)] = (uint32_t)pos;
+                    while (pos < f && (int32_t)(key[pos] >> 32) == p &&
+                           (int32_t)((key[pos] >> 26) & 63) == d)
+                        pos++;
+                }
+            }
+            toff[t][base + cnt - 1] = (uint32_t)pos;
+            if (pos != f) fail = 1;
+            tsrc[t].push_back(se);
+            e = f;
+        }
+    }
+    if (fail) {
+        err = "internal error: CSR row walk mismatch";
+        return -7;
+    }
+    size_t nsrc = 0, noff = 0;
+    for (int32_t t = 0; t < NT; t++) {
+        L.tile_src[t + 1] = L.tile_src[t] + (int32_t)tsrc[t].size();
+        nsrc += tsrc[t].size();
+        noff += toff[t].size();
+    }
+    L.src.reserve(nsrc);
+    L.off_tab.reserve(noff);
+    for (int32_t t = 0; t < NT; t++) {
+        const int64_t ob = (int64_t)L.off_tab.size();
+        for (auto se : tsrc[t]) {
+            se.off_base += ob;
+            L.src.push_back(se);
+        }
+        L.off_tab.insert(L.off_tab.end(), toff[t].begin(), toff[t].end());
+        std::vector<bsim_src_entry>().swap(tsrc[t]);
+        std::vector<uint32_t>().swap(toff[t]);
+    }
+    return 0;
+}
+
+}  // namespace bsim

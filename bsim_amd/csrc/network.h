@@ -1,0 +1,80 @@
+/*
+ * network.h -- host-side network store and the builder of the device layout.
+ *
+ * HostNet holds a network as the user built it through BSim's Network API
+ * (src/net/Network.h:120-209, src/net/Network.cpp:118-150): neurons in
+ * creation order with their own parameters, synapses in connect() order.
+ * Unlike the reference (one heap object per synapse, StaticSynapse.h:29-36)
+ * synapses are a compact structure of arrays (17 bytes each).
+ *
+ * build_layout() performs what Network::reset + Network::buildNetwork +
+ * copyNetworkToGPU's host half do in the reference (Network.cpp:286-306,
+ * BuildNetwork.cpp:36-205): per-neuron LIF constants, per-synapse delay steps
+ * and weight, and the tile-major (pre, delay) CSR described in layout.h, in
+ * O(S log S) per tile instead of the reference's O(S * D) scan.
+ */
+#ifndef BSIM_NETWORK_H
+#define BSIM_NETWORK_H
+
+#include <stdint.h>
+
+#include <string>
+#include <vector>
+
+#include "layout.h"
+
+namespace bsim {
+
+constexpr int kLifParams = 12;
+
+struct HostNet {
+    std::vector<int8_t> kind;
+    std::vector<float> lif;            /* n * kLifParams */
+    std::vector<int64_t> arr_off{0};   /* n + 1, into arr_t */
+    std::vector<int32_t> arr_t;
+    std::vector<uint32_t> bern_p;
+    uint64_t seed = 0;
+    std::vector<int32_t> pre, post;
+    std::vector<float> w, d;
+    std::vector<int8_t> stype;
+    std::vector<float> tau;            /* empty: every synapse has tau_in = 0 */
+
+    int32_t n() const { return (int32_t)kind.size(); }
+    int64_t s() const { return (int64_t)pre.size(); }
+};
+
+struct Layout {
+    int32_t n = 0, tile = 0, ntiles = 0, dmax = 1;
+    int64_t s = 0;
+    std::vector<int8_t> kind;
+    std::vector<float> Cm, CE, CI, C_E, C_I, v_tmp, v_thresh, v_reset, v_init;
+    std::vector<int32_t> refrac_time, arr_cur0, arr_end, arr_t;
+    std::vector<uint32_t> bern_p;
+    uint64_t seed = 0;
+    std::vector<int64_t> tile_syn;
+    std::vector<uint16_t> syn_tgt;
+    std::vector<float> syn_w;
+    std::vector<int32_t> tile_src;
+    std::vector<bsim_src_entry> src;
+    std::vector<uint32_t> off_tab;
+};
+
+/* LIFENeuron::reset (src/neuron/lif/LIFENeuron.cpp:123-167) for one neuron,
+ * in float, in the reference's operation order.  out: Cm CE CI C_E C_I v_tmp
+ * v_thresh v_reset v_init. */
+void lif_constants(const float *p, float dt, float out[9], int32_t *refrac_time);
+
+/* StaticSynapse::reset (src/synapse/static/StaticSynapse.cpp:81-92) with the
+ * tau picked as CompositeNeuron::createSynapse does (CompositeNeuron.h:200-218).
+ * Returns false where the reference asserts (non-integral delay, zero tau). */
+bool synapse_reset(float w, float delay, float tau, float dt, float *w_out, int32_t *ds);
+
+/* Chooses the tile size for n neurons on a device with `cus` compute units. */
+int32_t default_tile(int32_t n, int32_t cus);
+
+/* Returns 0 or a negative code with a message in err. */
+int build_layout(const HostNet &net, float dt, int32_t tile, Layout &out, std::string &err);
+
+}  // namespace bsim
+
+#endif

@@ -285,11 +285,12 @@ def random_net(seed, n_lif=40, n_arr=5, n_bern=3, p_conn=0.2, dmax=5, allow_dup=
         extra = rng.integers(0, len(pre), len(pre) // 5 + 1) if len(pre) else np.zeros(0, int)
         pre = np.concatenate([pre, pre[extra]])
         post = np.concatenate([post, post[extra]])
-    order = np.lexsort((post, pre))
-    pre, post = pre[order], post[order]
     m = len(pre)
     w = np.where(rng.random(m) < 0.3, -rng.uniform(0, 0.4, m), rng.uniform(0, 0.3, m))
     k = rng.integers(1, dmax + 1, m)
+    # creation order: pre-major, then target, duplicates in ascending delay
+    order = np.lexsort((k, post, pre))
+    pre, post, w, k = pre[order], post[order], w[order], k[order]
     if shuffle and m:
         perm = rng.permutation(m)
         pre, post, w, k = pre[perm], post[perm], w[perm], k[perm]

@@ -1,0 +1,179 @@
+/*
+ * bsim_amd.h -- C ABI of the MI355X simulation-step library (libbsim_amd.so).
+ *
+ * Plain C types only.  Every function returns 0 / a non-negative count on
+ * success and a negative BSIM_E* code on failure; bsim_last_error() then
+ * describes the failure (thread-local).  No function exits the process.
+ *
+ * Two groups of entry points:
+ *
+ * 1. Network construction on the host -- what the reference's Network class
+ *    stores when a program calls createPopulation()/addNeuron()/connect():
+ *      bsim_net_add_lif       <- Network::createPopulation<LIF_brian>,
+ *                                Population::addNeuron (src/net/Network.h:120-144,
+ *                                src/base/Population.h:158-168)
+ *      bsim_net_add_array     <- createPopulation<Array_spikes>
+ *                                (src/neuron/array/ArrayNeuron.cpp:70-77)
+ *      bsim_net_add_bernoulli <- (no counterpart: the reference's Poisson_spikes
+ *                                is seeded from the clock, PoissonNeuron.cpp:25-26)
+ *      bsim_net_connect       <- Network::connect(NeuronBase*, NeuronBase*, w, d,
+ *                                type, tau) applied to arrays of synapses
+ *                                (src/net/Network.cpp:118-150)
+ *
+ * 2. The device simulation -- the reference's GPU plug-in table and runtime:
+ *      bsim_sim_create  <- Network::reset + Network::buildNetwork +
+ *                          copyNetworkToGPU + alloc_buffers + getBlockSize
+ *                          (src/net/Network.cpp:286-306, src/net/BuildNetwork.cpp:36-205,
+ *                          src/gpu_utils/gpu_net_copy.cu:8-109, src/gpu_utils/GBuffers.cu:9-56,
+ *                          src/gpu_utils/gpu_func.cu:195-232; per type cudaAllocLIFE /
+ *                          cudaAllocStatic, src/utils/macros.h:37-58)
+ *      bsim_sim_step    <- one cycle of SingleGPUSimulator::run's loop:
+ *                          cudaUpdateLIFE + cudaUpdateStatic + update_time
+ *                          (src/sim/SingleGPUSimulator.cu:146-245, src/gpu_utils/gpu_func.cu:
+ *                          149-179, src/gpu_utils/runtime.cu:81-92), n cycles per call
+ *      bsim_sim_read_*  <- cudaFetchLIFE / fetchNetworkFromGPU, the GSim.log raster and
+ *                          g_v.data trace, gFireCount ("count" of FireInfo)
+ *                          (src/neuron/lif/GLIFENeurons.cu:29-40, src/gpu_utils/gpu_net_copy.cu:
+ *                          111-132, src/sim/SingleGPUSimulator.cu:176-296)
+ *      bsim_sim_destroy <- free_buffers + freeGPUNetwork (GBuffers.cu:64-78,
+ *                          gpu_net_copy.cu:134-191)
+ */
+#ifndef BSIM_AMD_H
+#define BSIM_AMD_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define BSIM_ABI_VERSION 1
+
+enum {
+    BSIM_OK = 0,
+    BSIM_E_ARG = -1,        /* invalid argument */
+    BSIM_E_NETWORK = -2,    /* network not supported by the GPU step (message says why) */
+    BSIM_E_NOMEM = -3,      /* host or device allocation failed */
+    BSIM_E_HIP = -4,        /* HIP runtime error */
+    BSIM_E_STATE = -5,      /* call not valid in the object's current state */
+    BSIM_E_CAPACITY = -6    /* caller buffer or recording window too small */
+};
+
+typedef struct bsim_network bsim_network;
+typedef struct bsim_sim bsim_sim;
+
+int32_t bsim_abi_version(void);
+const char *bsim_last_error(void);
+
+/* ---- network construction (host) ---------------------------------------- */
+
+bsim_network *bsim_net_create(void);
+void bsim_net_destroy(bsim_network *net);
+
+/* Appends n LIF_brian neurons; params is n rows of 12 floats:
+ * v_init v_rest v_reset cm tau_m tau_refrac tau_syn_E tau_syn_I v_thresh
+ * i_offset, then the composite's own tau_syn_E tau_syn_I.  Returns the global
+ * id of the first neuron added (ids are creation order). */
+int32_t bsim_net_add_lif(bsim_network *net, int32_t n, const float *params);
+
+/* Appends n array sources; neuron i fires at cycles
+ * cycles[offsets[i] .. offsets[i+1]) (ascending, offsets[0] == 0).
+ * taus (may be NULL) holds the composite tau_syn_E/I per neuron. */
+int32_t bsim_net_add_array(bsim_network *net, int32_t n, const int64_t *offsets,
+                           const int32_t *cycles, const float *taus);
+
+/* Appends n seeded Bernoulli sources firing at cycle t iff
+ * hash(seed, id, t) < p[i]; the seed is shared by the network. */
+int32_t bsim_net_add_bernoulli(bsim_network *net, int32_t n, const uint32_t *p, uint64_t seed);
+
+/* Appends count synapses in the given order (= creation order).  d is the
+ * delay in seconds; stype (NULL: all excitatory) and tau (NULL: all 0) as in
+ * Network::connect.  Returns the number of synapses now in the network. */
+int64_t bsim_net_connect(bsim_network *net, int64_t count, const int32_t *pre,
+                         const int32_t *post, const float *w, const float *d,
+                         const int8_t *stype, const float *tau);
+
+int32_t bsim_net_num_neurons(const bsim_network *net);
+int64_t bsim_net_num_synapses(const bsim_network *net);
+
+/* ---- device simulation ---------------------------------------------------- */
+
+typedef struct bsim_sim_config {
+    int32_t device;        /* HIP device ordinal */
+    int32_t tile;          /* neurons per tile (multiple of 64); 0 = automatic */
+    int32_t rec_steps;     /* cycles of raster/trace kept on the device (0 = none) */
+    int32_t n_rec;         /* neurons whose V is traced */
+    const int32_t *rec_ids;
+} bsim_sim_config;
+
+/* Resets the network (LIF constants, synapse delays), builds the device
+ * layout, uploads it and allocates the step buffers.  The network may be
+ * destroyed afterwards. */
+bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_config *cfg);
+
+/* Advances n cycles (asynchronous on the simulation's stream). */
+int32_t bsim_sim_step(bsim_sim *sim, int32_t n);
+
+/* Performs phase (ii) of the last cycle, which the next cycle would consume;
+ * run(T) does this once at the end so that every delivery of the T cycles is
+ * executed, as in the reference.  The neuron state is not changed. */
+int32_t bsim_sim_finish(bsim_sim *sim);
+
+int32_t bsim_sim_sync(bsim_sim *sim);
+
+/* Cycles simulated so far. */
+int32_t bsim_sim_cycle(const bsim_sim *sim);
+
+/* Raster of the recorded cycles as ascending fired ids per cycle: counts has
+ * one entry per recorded cycle (min(cycle, rec_steps)); ids receives the
+ * concatenation.  Returns the number of ids, or BSIM_E_CAPACITY (with the
+ * needed size in *needed) if cap is too small. */
+int64_t bsim_sim_read_raster(bsim_sim *sim, int32_t *counts, int32_t *ids, int64_t cap,
+                             int64_t *needed);
+
+/* V trace of the recorded neurons: [recorded cycles][n_rec]. */
+int32_t bsim_sim_read_trace(bsim_sim *sim, float *v);
+
+/* Neuron state (any pointer may be NULL): V, i_E, i_I, refractory counter,
+ * spike count. */
+int32_t bsim_sim_read_state(bsim_sim *sim, float *v, float *ie, float *ii, int32_t *refrac,
+                            int32_t *fire_count);
+
+/* Synaptic deliveries performed so far (sum over tiles). */
+int64_t bsim_sim_events(bsim_sim *sim);
+
+typedef struct bsim_sim_info {
+    int32_t n, tile, ntiles, dmax;
+    int64_t s;
+    int64_t device_bytes;
+    int64_t n_src_entries;
+    int64_t off_entries;
+} bsim_sim_info;
+
+int32_t bsim_sim_get_info(const bsim_sim *sim, bsim_sim_info *info);
+
+/* The simulation's HIP stream (hipStream_t) for callers that time or order
+ * work around the step. */
+void *bsim_sim_stream(bsim_sim *sim);
+
+void bsim_sim_destroy(bsim_sim *sim);
+
+/* ---- host-side inspection (no device needed) ------------------------------ */
+
+/* LIF constants as LIFENeuron::reset computes them: per neuron Cm CE CI C_E
+ * C_I v_tmp v_thresh v_reset v_init (9 floats) and refrac_time. */
+int32_t bsim_net_lif_constants(const bsim_network *net, float dt, float *out9,
+                               int32_t *refrac_time);
+
+/* Device layout computed on the host, for inspection: per tile synapse
+ * ranges and the (pre, delay)-sorted local targets / weights.  Pass NULL
+ * arrays to query sizes through *ntiles / *s. */
+int32_t bsim_net_layout(const bsim_network *net, float dt, int32_t tile, int32_t *ntiles,
+                        int64_t *s, int32_t *dmax, int64_t *tile_syn, uint16_t *syn_tgt,
+                        float *syn_w);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

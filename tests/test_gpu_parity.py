@@ -1,0 +1,134 @@
+"""Parity of the HIP step with the reference (golden vectors from the
+reference's own CPU simulator) and with the oracle, through the C ABI.
+
+Bar: rasters equal as ordered id lists per cycle; V traces, final state and
+per-neuron spike counts equal bit for bit; delivered-event counts equal.
+"""
+import sys
+
+import numpy as np
+import pytest
+
+import oracle
+from bsim_amd import api, netspec
+
+import golden_io
+
+pytestmark = pytest.mark.gpu
+DT = 1e-4
+
+
+def say(*a):
+    print(*a, flush=True)
+    sys.stdout.flush()
+
+
+def run_gpu(spec, steps, rec_ids, tile=0):
+    net = api.Network.from_spec(spec)
+    sim = api.Simulation(net, DT, tile=tile, rec_steps=steps, rec_ids=rec_ids)
+    sim.step(steps)
+    sim.finish()
+    sim.sync()
+    counts, ids, _ = sim.raster()
+    out = dict(raster_count=counts, raster_ids=ids, rec_v=sim.trace(), events=sim.events(),
+               info=sim.info(), **sim.state())
+    sim.close()
+    return out
+
+
+def assert_same(g, o, what):
+    np.testing.assert_array_equal(g["raster_count"], o.raster_count, err_msg=f"{what}: counts")
+    np.testing.assert_array_equal(g["raster_ids"], o.raster_ids, err_msg=f"{what}: ids")
+    assert np.array_equal(g["rec_v"].view(np.uint32), o.rec_v.view(np.uint32)), f"{what}: V"
+    lif = slice(None)
+    assert np.array_equal(g["v"].view(np.uint32)[lif], o.final_v.view(np.uint32)[lif]), what
+    assert np.array_equal(g["ie"].view(np.uint32), o.final_ie.view(np.uint32)), what
+    assert np.array_equal(g["ii"].view(np.uint32), o.final_ii.view(np.uint32)), what
+    np.testing.assert_array_equal(g["refrac"], o.final_refrac, err_msg=what)
+    np.testing.assert_array_equal(g["fire_count"], o.fire_count, err_msg=what)
+    assert g["events"] == o.events, (what, g["events"], o.events)
+
+
+GOLDEN_F32 = [n for n in golden_io.case_names() if n.endswith("_f32")]
+
+
+@pytest.mark.parametrize("tile", [0, 64])
+@pytest.mark.parametrize("name", GOLDEN_F32)
+def test_golden_cases(name, tile):
+    """GPU == reference CPU build, bit for bit."""
+    meta, spec, dtype, g = golden_io.load(name)
+    say(f"[gpu] golden {name} tile={tile} n={spec.n} s={spec.s} steps={meta['steps']}")
+    r = run_gpu(spec, meta["steps"], meta["rec_ids"], tile)
+    np.testing.assert_array_equal(r["raster_count"], g["raster_count"])
+    np.testing.assert_array_equal(r["raster_ids"], g["raster_ids"])
+    assert np.array_equal(r["rec_v"].view(np.uint32), g["rec_v_bits"])
+    np.testing.assert_array_equal(r["fire_count"], g["fire_count"])
+    lif = spec.kind == netspec.LIF
+    assert np.array_equal(r["v"].view(np.uint32)[lif], g["consts_bits"][lif][:, 8])
+    o = oracle.run(spec, DT, meta["steps"], rec_ids=meta["rec_ids"])
+    assert r["events"] == o.events
+
+
+@pytest.mark.parametrize("seed", range(6))
+@pytest.mark.parametrize("tile", [64, 128, 0])
+def test_random_networks_vs_oracle(seed, tile):
+    spec = netspec.random_net(100 + seed, n_lif=300, n_arr=20, n_bern=20, p_conn=0.05,
+                              dmax=1 + 7 * seed, allow_dup=seed % 2 == 0, spike_prob=0.1)
+    steps = 400
+    rec = list(range(0, spec.n, 7))
+    say(f"[gpu] random seed={seed} tile={tile} n={spec.n} s={spec.s}")
+    r = run_gpu(spec, steps, rec, tile)
+    o = oracle.run(spec, DT, steps, rec_ids=rec)
+    assert o.raster_ids.size > 100
+    assert_same(r, o, f"random{seed}")
+
+
+def test_max_delay_63():
+    spec = netspec.standard(npop=3, n=300, fanout=40, dmax=63, w0=4.0, drive_rate_hz=300.0,
+                            n_drive=100, drive_w=14.0, seed=5, i_offset=(12.0, 14.5))
+    say(f"[gpu] dmax63 n={spec.n} s={spec.s}")
+    r = run_gpu(spec, 500, [150, 450, 750], 64)
+    o = oracle.run(spec, DT, 500, rec_ids=[150, 450, 750])
+    assert r["info"]["dmax"] == 63
+    assert_same(r, o, "dmax63")
+
+
+def test_standard_medium_vs_csr_oracle():
+    """A C1-shaped network at 1/40 scale (16 x 1000, fan-out 250, delays
+    1..6, Bernoulli drive): GPU vs the oracle's CSR form on 8 threads."""
+    spec = netspec.standard(npop=16, n=1000, fanout=250, w0=2.0, inh_frac=0.25, dmax=6,
+                            drive_rate_hz=500.0, n_drive=500, drive_w=10.0, seed=11,
+                            i_offset=(10.0, 14.9))
+    steps = 1000
+    rec = [500, 1500, 9000, 16499]
+    say(f"[gpu] standard medium n={spec.n} s={spec.s}")
+    r = run_gpu(spec, steps, rec)
+    o = oracle.run(spec, DT, steps, rec_ids=rec, mode="csr", tile=512, threads=8)
+    assert o.events > 1_000_000
+    assert_same(r, o, "standard-medium")
+
+
+def test_run_twice_identical():
+    spec = netspec.standard(npop=4, n=500, fanout=80, w0=3.0, drive_rate_hz=400.0, n_drive=200,
+                            drive_w=12.0, seed=3, i_offset=(11.0, 14.0))
+    a = run_gpu(spec, 300, [10, 900])
+    b = run_gpu(spec, 300, [10, 900])
+    for k in ("raster_ids", "fire_count"):
+        np.testing.assert_array_equal(a[k], b[k])
+    assert np.array_equal(a["rec_v"].view(np.uint32), b["rec_v"].view(np.uint32))
+    assert a["events"] == b["events"]
+
+
+def test_step_in_chunks_equals_one_call():
+    spec = netspec.standard(npop=3, n=400, fanout=60, w0=3.0, drive_rate_hz=400.0, n_drive=100,
+                            drive_w=12.0, seed=9, i_offset=(11.0, 14.0))
+    net = api.Network.from_spec(spec)
+    s1 = api.Simulation(net, DT, rec_steps=200, rec_ids=[5, 700])
+    for n in (1, 2, 3, 50, 144):
+        s1.step(n)
+    s1.sync()
+    s2 = api.Simulation(net, DT, rec_steps=200, rec_ids=[5, 700])
+    s2.step(200)
+    s2.sync()
+    np.testing.assert_array_equal(s1.raster()[1], s2.raster()[1])
+    assert np.array_equal(s1.trace().view(np.uint32), s2.trace().view(np.uint32))

@@ -1,0 +1,118 @@
+"""Host-side checks of the product library (no GPU calls): it loads, exports
+every symbol include/bsim_amd.h declares, computes the reference's constants
+bit for bit, lays synapses out in (pre, delay, creation) order per tile, and
+rejects networks the GPU step cannot run with a clear error."""
+import os
+import re
+
+import numpy as np
+import pytest
+
+import oracle
+from bsim_amd import api, netspec
+
+import golden_io
+
+ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
+DT = 1e-4
+
+
+def header_functions():
+    src = open(os.path.join(ROOT, "include", "bsim_amd.h")).read()
+    src = re.sub(r"/\*.*?\*/", "", src, flags=re.S)
+    return sorted(set(re.findall(r"\b(bsim_[a-z0-9_]+)\s*\(", src)))
+
+
+def test_library_exports_header_symbols():
+    L = api.lib()
+    names = header_functions()
+    assert len(names) >= 20
+    for name in names:
+        assert hasattr(L, name), name
+    assert set(names) == set(api._SIGS), set(names) ^ set(api._SIGS)
+    assert L.bsim_abi_version() == 1
+
+
+@pytest.mark.parametrize("name", [n for n in golden_io.case_names() if n.endswith("_f32")])
+def test_constants_match_reference(name):
+    meta, spec, dtype, g = golden_io.load(name)
+    net = api.Network.from_spec(spec)
+    c9, rt = net.lif_constants(meta["dt"])
+    lif = spec.kind == netspec.LIF
+    assert np.array_equal(c9[:, :8].copy().view(np.uint32)[lif], g["consts_bits"][lif][:, :8])
+    np.testing.assert_array_equal(rt[lif], g["refrac_time"][lif])
+
+
+def test_layout_is_pre_delay_sorted():
+    spec = netspec.random_net(5, n_lif=150, p_conn=0.15, dmax=7, allow_dup=True)
+    net = api.Network.from_spec(spec)
+    tile = 64
+    L = net.layout(DT, tile)
+    w_ref, ds = oracle.synapse_reset(spec, DT)
+    for k in range(L["ntiles"]):
+        sel = np.nonzero(spec.post // tile == k)[0]          # creation order
+        order = sel[np.lexsort((sel, ds[sel], spec.pre[sel]))]
+        a, b = L["tile_syn"][k], L["tile_syn"][k + 1]
+        np.testing.assert_array_equal(L["syn_tgt"][a:b], spec.post[order] - k * tile)
+        assert np.array_equal(L["syn_w"][a:b].view(np.uint32), w_ref[order].view(np.uint32))
+
+
+def test_layout_single_tile_matches_reference_csr():
+    """test/lif_unit1_test.py:105-106: p_dst = [10..19] x 10 + [20..29] x 10."""
+    net = api.Network.from_spec(netspec.lif_unit(1))
+    L = net.layout(DT, 64)
+    assert L["ntiles"] == 1 and L["dmax"] == 1
+    np.testing.assert_array_equal(L["syn_tgt"], list(range(10, 20)) * 10 + list(range(20, 30)) * 10)
+    np.testing.assert_array_equal(L["syn_w"], np.float32([1e-5] * 100 + [2e-5] * 100))
+
+
+def _two_neuron(delay_steps, w=0.5, tau=None, dexact=True):
+    b = netspec.Builder()
+    b.population(np.tile(netspec.lif_row(0, 0, 0, 0.02, 0.02, 0, 5e-3, 5e-3, 1.0, 0.5), (2, 1)))
+    d = (np.float32(delay_steps) * np.float32(DT)) if dexact else np.float32(delay_steps * DT * 1.3)
+    b.connect([0], [1], np.float32([w]), np.float32([d]), tau=None if tau is None else [tau])
+    return b.build()
+
+
+@pytest.mark.parametrize("steps,msg", [(0, "outside 1..63"), (64, "outside 1..63")])
+def test_rejects_unsupported_delay(steps, msg):
+    net = api.Network.from_spec(_two_neuron(steps))
+    with pytest.raises(api.BsimError, match=msg):
+        net.layout(DT, 64)
+
+
+def test_rejects_nonintegral_delay_without_tau():
+    spec = _two_neuron(2, dexact=False)
+    spec.tau = np.zeros(1, np.float32)
+    spec.lif[:, 10:12] = 0.0   # composite taus zero: the reference asserts
+    net = api.Network.from_spec(spec)
+    with pytest.raises(api.BsimError, match="not a whole number of steps"):
+        net.layout(DT, 64)
+
+
+def test_nonintegral_delay_rescales_weight_like_reference():
+    spec = _two_neuron(2, dexact=False)
+    net = api.Network.from_spec(spec)
+    L = net.layout(DT, 64)
+    w, ds = oracle.synapse_reset(spec, DT)
+    assert L["syn_w"][0].view(np.uint32) == w[0].view(np.uint32)
+    assert L["dmax"] == ds[0]
+
+
+def test_rejects_creation_order_against_pre_order():
+    spec = netspec.random_net(1, shuffle=True, n_lif=60, p_conn=0.3)
+    net = api.Network.from_spec(spec)
+    with pytest.raises(api.BsimError, match="ascending \\(pre, delay\\) order"):
+        net.layout(DT, 64)
+
+
+def test_bad_arguments():
+    net = api.Network()
+    with pytest.raises(api.BsimError):
+        net.add_array(np.array([1, 2], np.int64), np.array([0], np.int32))
+    with pytest.raises(api.BsimError, match="multiple of 64"):
+        api.Network.from_spec(netspec.lif_unit(1)).layout(DT, 100)
+    with pytest.raises(api.BsimError, match="outside the network"):
+        n2 = api.Network.from_spec(netspec.lif_unit(1))
+        n2.connect([0], [999], [0.1], [1e-4])
+        n2.layout(DT, 64)
