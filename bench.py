@@ -1,0 +1,229 @@
+"""Benchmark of the simulation step on one MI355X.
+
+    python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c0]
+                    [--cpu-steps C] [--no-cpu]
+
+A "step" is one simulation cycle of the whole network (the reference's
+SGSim loop body: LIF update, spike compaction, delayed delivery, synaptic
+accumulation).  The default workload is the standard feed-forward network
+at ~100 Hz (64 x 10,000 LIF, ~6.3e8 synapses, delays 1..6 steps, dt 0.1 ms),
+built from the seeded generator of bsim_amd.netspec and uploaded before the
+timed region.  W cycles run untimed, then exactly K cycles are timed between
+two stream synchronisations.  One JSON line is printed:
+
+  value        synaptic events (deliveries) per second over the K cycles
+  roofline     the step kernel's algorithmic bytes (8 B per delivered event:
+               fp32 weight + int32 target, as the reference stores them) per
+               launch over its mean launch duration, measured with HIP events
+               recorded around each launch on the simulation's stream during a
+               further 200-cycle pass; peak = 8 TB/s (MI355X HBM3E)
+  cpu_baseline the oracle's event-driven CSR form (oracle/stsim.c) on the
+               host's cores, timed over a window of the same network after
+               the same warm-up
+
+Rasters are not recorded in the timed region (recording costs one 8-byte
+store per 64 neurons per cycle); bit-exactness is established by the gpu
+tests on the same code path.
+"""
+from __future__ import annotations
+
+import argparse
+import ctypes as C
+import json
+import os
+import platform
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+from bsim_amd import api, netspec  # noqa: E402
+
+HBM_PEAK_GBS = 8000.0
+BYTES_PER_EVENT = 8
+
+
+class HipEvents:
+    """hipEvent timing on an arbitrary stream (ctypes on the HIP runtime)."""
+
+    def __init__(self):
+        for name in ("libamdhip64.so", "libamdhip64.so.7", "/opt/rocm/lib/libamdhip64.so"):
+            try:
+                self.h = C.CDLL(name)
+                break
+            except OSError:
+                continue
+        else:
+            raise RuntimeError("libamdhip64 not found")
+        self.h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
+        self.h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
+        self.h.hipEventSynchronize.argtypes = [C.c_void_p]
+        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
+        self.h.hipEventDestroy.argtypes = [C.c_void_p]
+
+    def create(self, n):
+        evs = []
+        for _ in range(n):
+            e = C.c_void_p()
+            if self.h.hipEventCreate(C.byref(e)) != 0:
+                raise RuntimeError("hipEventCreate failed")
+            evs.append(e)
+        return evs
+
+    def record(self, e, stream):
+        if self.h.hipEventRecord(e, C.c_void_p(stream)) != 0:
+            raise RuntimeError("hipEventRecord failed")
+
+    def ms(self, a, b):
+        self.h.hipEventSynchronize(b)
+        f = C.c_float()
+        if self.h.hipEventElapsedTime(C.byref(f), a, b) != 0:
+            raise RuntimeError("hipEventElapsedTime failed")
+        return float(f.value)
+
+    def destroy(self, evs):
+        for e in evs:
+            self.h.hipEventDestroy(e)
+
+
+def host_info():
+    model = "unknown"
+    try:
+        for line in open("/proc/cpuinfo"):
+            if line.startswith("model name"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except OSError:
+        pass
+    return dict(cpu=model, nproc=os.cpu_count(), machine=platform.machine())
+
+
+def make_workload(name):
+    if name == "c1":
+        return netspec.c1(), "standard feed-forward ~100 Hz: 64 x 10,000 LIF_brian, ~1,000 " \
+                             "syn/neuron, delays 1..6, dt 0.1 ms"
+    if name == "c2":
+        return netspec.c2(), "high-activity stress: 100 x 10,000 LIF_brian, ~2,000 syn/neuron, " \
+                             "delays 1..50, dt 0.1 ms"
+    if name == "c0":
+        return netspec.c0(), "plain method: 2 x 1,000 LIF_brian, 1e6 all-to-all synapses"
+    if name.startswith("c1x"):   # c1x<npop>: the C1 generator with fewer populations
+        k = int(name[3:])
+        return netspec.c1(npop=k), f"standard feed-forward, {k} x 10,000 LIF_brian"
+    raise SystemExit(f"unknown workload {name}")
+
+
+def cpu_baseline(spec, warmup, steps, threads):
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # noqa: E402  (the oracle is only the checker / baseline)
+    r = oracle.run(spec, netspec.DT, warmup + steps, mode="csr", tile=2048, threads=threads,
+                   raster_cap=1 << 26, time_from=warmup)
+    return r.events_from / r.wall_s, r.events_from, r.wall_s
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10000)
+    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--workload", default="c1")
+    ap.add_argument("--tile", type=int, default=0)
+    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--roofline-steps", type=int, default=200)
+    a = ap.parse_args()
+    if a.gpus != 1:
+        print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU spike "
+              "exchange is not part of this build.", file=sys.stderr)
+        sys.exit(2)
+
+    t0 = time.time()
+    spec, desc = make_workload(a.workload)
+    t_gen = time.time() - t0
+    print(f"[bench] workload {a.workload}: n={spec.n} s={spec.s} (generated in {t_gen:.1f}s)",
+          file=sys.stderr, flush=True)
+    t0 = time.time()
+    net = api.Network.from_spec(spec)
+    sim = api.Simulation(net, netspec.DT, tile=a.tile)
+    del net
+    t_build = time.time() - t0
+    info = sim.info()
+    print(f"[bench] built+uploaded in {t_build:.1f}s: {info}", file=sys.stderr, flush=True)
+
+    sim.step(a.warmup)
+    sim.sync()
+    ev0 = sim.events()
+    t0 = time.perf_counter()
+    sim.step(a.steps)
+    sim.sync()
+    t1 = time.perf_counter()
+    ev1 = sim.events()
+    wall = t1 - t0
+    events = ev1 - ev0
+    value = events / wall
+    print(f"[bench] timed {a.steps} cycles in {wall:.4f}s, {events} events", file=sys.stderr,
+          flush=True)
+
+    # live kernel timing: an event pair around every launch of a further pass
+    hip = HipEvents()
+    stream = sim.stream()
+    R = a.roofline_steps
+    evs = hip.create(2 * R)
+    evr0 = sim.events()
+    for i in range(R):
+        hip.record(evs[2 * i], stream)
+        sim.step(1)
+        hip.record(evs[2 * i + 1], stream)
+    sim.sync()
+    evr1 = sim.events()
+    durs = np.array([hip.ms(evs[2 * i], evs[2 * i + 1]) for i in range(R)])
+    hip.destroy(evs)
+    mean_ms = float(durs.mean())
+    ev_per_launch = (evr1 - evr0) / R
+    achieved = ev_per_launch * BYTES_PER_EVENT / (mean_ms * 1e-3) / 1e9
+    print(f"[bench] kernel mean {mean_ms * 1e3:.2f} us over {R} launches, "
+          f"{ev_per_launch:.0f} events/launch", file=sys.stderr, flush=True)
+    sim.close()
+
+    cpu = None
+    if not a.no_cpu:
+        threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
+        cw = min(a.warmup, 1000)
+        v, ev, secs = cpu_baseline(spec, cw, a.cpu_steps, threads)
+        cpu = {"value": v, "unit": "synaptic events/s", "cores": threads, "kind": "port",
+               "sample": f"oracle CSR form, cycles {cw}..{cw + a.cpu_steps} of the same "
+                         f"network ({ev} events in {secs:.2f}s)", "host": host_info()}
+        print(f"[bench] cpu baseline {v:.3g} events/s on {threads} threads", file=sys.stderr,
+              flush=True)
+
+    out = {
+        "metric": "synaptic events/s",
+        "value": value,
+        "unit": "events/s",
+        "n_gpus": 1,
+        "steps": a.steps,
+        "warmup": a.warmup,
+        "ms_per_step": wall / a.steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded generator)",
+        "config": {"workload": desc, "name": a.workload, "neurons": spec.n,
+                   "synapses": spec.s, "dt_s": netspec.DT, "tile": info["tile"],
+                   "simulated_s": a.steps * netspec.DT, "parallelism": "single GPU"},
+        "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "kernel": "bsim_step_kernel", "mean_launch_us": mean_ms * 1e3,
+                     "events_per_launch": ev_per_launch},
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

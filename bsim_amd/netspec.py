@@ -186,76 +186,120 @@ def c0(seed: int = 1234, n: int = 1000, tau_syn_E=5e-3, tau_syn_I=10e-3) -> NetS
 
 
 def standard(npop=4, n=1000, fanout=100, w0=1.0, inh_frac=0.25, dmax=6, remote_frac=0.2,
-             drive_rate_hz=0.0, n_drive=None, drive_w=None, seed=1, tau_refrac=2e-3,
-             i_offset=(0.0, 0.0), tau_syn_E=5e-3, tau_syn_I=10e-3, v_thresh=15.0):
+             drive_rate_hz=0.0, n_drive=None, drive_w=None, drive_fanout=None, seed=1,
+             tau_refrac=2e-3, i_offset=(0.0, 0.0), tau_syn_E=5e-3, tau_syn_I=10e-3,
+             v_thresh=15.0):
     """Feed-forward chain of ``npop`` LIF populations of ``n`` neurons.
 
     Population i projects to i+1; a ``remote_frac`` share of populations also
-    project to one uniformly chosen remote population (not i, not i+1).  Each
-    pre neuron has ``fanout`` distinct targets per projection, drawn by
-    stratified sampling (one target in each of ``fanout`` equal strata).
+    project to one uniformly chosen remote population (not i, not i+1) and
+    split their ``fanout`` out-synapses per neuron evenly between the two
+    projections.  Targets of a pre neuron within a projection are distinct,
+    drawn by stratified sampling (one target in each of f equal strata).
     Excitatory weights ~ U[0, 0.7 w0), a share ``inh_frac`` inhibitory with
-    weights ~ U[-0.5 w0, 0); integer delays U{1..dmax} steps.  Population 0 is
-    driven by ``n_drive`` Bernoulli sources firing at ``drive_rate_hz`` (one
-    projection drive -> pop 0 with the same fan-out rule).  Synapses are
-    created source population by source population, pre-major, so every
-    target's inputs arrive in ascending (pre, delay) creation order.
+    weights ~ U[-0.5 w0, 0); integer delays U{1..dmax} steps.  LIF neurons:
+    tau_m 20 ms, R = 1, v_rest = v_reset = 0, i_offset ~ U[i_offset).
+    Population 0 is driven by ``n_drive`` Bernoulli sources firing at
+    ``drive_rate_hz`` (projection drive -> pop 0, ``drive_fanout`` targets
+    each, weight ``drive_w``).  Synapses are created source population by
+    source population, pre-major, so every target's inputs arrive in
+    ascending (pre, delay) creation order.
     """
     rng = np.random.Generator(np.random.PCG64(seed))
-    b = Builder()
     tau_m = 20e-3
+    kinds, rows_l, bern_l, pops = [], [], [], []
+    nn = 0
     drive = None
     if drive_rate_hz > 0:
         nd = n if n_drive is None else n_drive
         p = min(1.0, drive_rate_hz * DT)
         thr = np.uint32(min(0xFFFFFFFF, int(round(p * 4294967296.0))))
-        drive = b.population(np.zeros((nd, LIF_NPARAM)), kind=BERNOULLI, bern_p=thr)
-    pops = []
+        kinds.append(np.full(nd, BERNOULLI, np.int8))
+        rows_l.append(np.zeros((nd, LIF_NPARAM)))
+        bern_l.append(np.full(nd, thr, np.uint32))
+        drive = 0
+        pops.append((0, nd))
+        nn = nd
+    first_lif = len(pops)
     for _ in range(npop):
         rows = np.tile(lif_row(0.0, 0.0, 0.0, tau_m, tau_m, tau_refrac, tau_syn_E, tau_syn_I,
                                v_thresh, 0.0), (n, 1))
         if i_offset[1] > i_offset[0]:
             rows[:, 9] = rng.uniform(i_offset[0], i_offset[1], n).astype(np.float32)
-        pops.append(b.population(rows))
-    # projections: (src pop index in b, dst pop index in b)
-    proj = []
-    if drive is not None:
-        proj.append((drive, pops[0]))
+        kinds.append(np.full(n, LIF, np.int8))
+        rows_l.append(rows)
+        bern_l.append(np.zeros(n, np.uint32))
+        pops.append((nn, n))
+        nn += n
     remotes = {}
     for i in range(npop):
         if rng.random() < remote_frac and npop > 2:
             choices = [j for j in range(npop) if j != i and j != i + 1]
             remotes[i] = int(rng.choice(choices))
+    proj = []   # (src pop index, dst pop index, fan-out)
+    if drive is not None:
+        proj.append((drive, first_lif, min(fanout if drive_fanout is None else drive_fanout, n)))
     for i in range(npop):
+        src = first_lif + i
         if i + 1 < npop:
-            proj.append((pops[i], pops[i + 1]))
+            f = fanout - fanout // 2 if i in remotes else fanout
+            proj.append((src, src + 1, min(f, n)))
         if i in remotes:
-            proj.append((pops[i], pops[remotes[i]]))
+            proj.append((src, first_lif + remotes[i], min(fanout // 2, n)))
     proj.sort(key=lambda x: x[0])  # creation: ascending source population
-    for src, dst in proj:
-        s0, sn = b.pops[src]
-        d0, dn = b.pops[dst]
-        f = min(fanout, dn)
+    total = sum(pops[a][1] * f for a, _, f in proj)
+    pre = np.empty(total, np.int32)
+    post = np.empty(total, np.int32)
+    w = np.empty(total, np.float32)
+    d = np.empty(total, np.float32)
+    st = np.empty(total, np.int8)
+    o = 0
+    for src, dst, f in proj:
+        s0, sn = pops[src]
+        d0, dn = pops[dst]
+        m = sn * f
         strata = dn / f
-        pre = np.repeat(np.arange(s0, s0 + sn, dtype=np.int64), f)
+        pre[o:o + m] = np.repeat(np.arange(s0, s0 + sn, dtype=np.int32), f)
         base = (np.arange(f) * strata)[None, :]
         off = np.floor(base + rng.random((sn, f)) * strata).astype(np.int64)
-        off = np.minimum(off, dn - 1).reshape(-1)
-        post = d0 + off
-        m = sn * f
+        post[o:o + m] = d0 + np.minimum(off, dn - 1).reshape(-1)
         if src == drive:
-            wv = (np.full(m, 0.7 * w0) if drive_w is None else np.full(m, drive_w))
-            st = np.zeros(m, np.int8)
+            w[o:o + m] = np.float32(0.7 * w0 if drive_w is None else drive_w)
+            st[o:o + m] = 0
         else:
             inh = rng.random(m) < inh_frac
-            wv = np.where(inh, rng.uniform(-0.5 * w0, 0.0, m), rng.uniform(0.0, 0.7 * w0, m))
-            st = inh.astype(np.int8)
+            w[o:o + m] = np.where(inh, rng.uniform(-0.5 * w0, 0.0, m),
+                                  rng.uniform(0.0, 0.7 * w0, m)).astype(np.float32)
+            st[o:o + m] = inh
         k = rng.integers(1, dmax + 1, m)
-        b.connect(pre, post, wv.astype(np.float32), k.astype(np.float32) * np.float32(DT),
-                  st)
-    spec = b.build()
-    spec.seed = seed
+        d[o:o + m] = k.astype(np.float32) * np.float32(DT)
+        o += m
+    spec = NetSpec(kind=np.concatenate(kinds), lif=np.concatenate(rows_l), pre=pre, post=post,
+                   w=w, d=d, stype=st, tau=None, bern_p=np.concatenate(bern_l), seed=seed,
+                   pops=pops)
     return spec
+
+
+# The standard feed-forward workload at ~100 Hz: 64 x 10,000 LIF, ~1,000
+# out-synapses per neuron, 20 % remote projections, 25 % inhibitory, delays
+# 1..6 steps; neurons carry i_offset ~ U[14.5, 16.5) (as the plain method)
+# and population 0 receives a Bernoulli drive.
+C1 = dict(npop=64, n=10000, fanout=1000, w0=0.45, inh_frac=0.25, dmax=6, remote_frac=0.2,
+          drive_rate_hz=20.0, n_drive=10000, drive_w=1.0, drive_fanout=100, seed=1,
+          i_offset=(14.5, 16.5))
+# High-activity stress workload: 100 x 10,000 LIF, ~2,000 out-synapses per
+# neuron, w0 x 1.3, 20 % inhibitory, delays 1..50 steps, no refractory period.
+C2 = dict(npop=100, n=10000, fanout=2000, w0=0.45 * 1.3, inh_frac=0.2, dmax=50,
+          remote_frac=0.2, drive_rate_hz=20.0, n_drive=10000, drive_w=1.0, drive_fanout=100,
+          seed=2, i_offset=(14.5, 16.5), tau_refrac=0.0)
+
+
+def c1(**over):
+    return standard(**{**C1, **over})
+
+
+def c2(**over):
+    return standard(**{**C2, **over})
 
 
 def random_net(seed, n_lif=40, n_arr=5, n_bern=3, p_conn=0.2, dmax=5, allow_dup=False,

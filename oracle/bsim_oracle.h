@@ -81,6 +81,9 @@ typedef struct bso_out_##SUF {                                                \
     REAL *final_ii;          /* [n]                (may be NULL)            */ \
     int32_t *final_refrac;   /* [n]                (may be NULL)            */ \
     int64_t events;          /* synaptic deliveries performed               */ \
+    int32_t time_from;       /* cycle from which wall time is measured      */ \
+    double wall_s;           /* seconds spent in cycles >= time_from        */ \
+    int64_t events_from;     /* deliveries in cycles >= time_from           */ \
 } bso_out_##SUF;                                                              \
                                                                               \
 int bso_lif_constants_##SUF(const bso_net_##SUF *net, REAL dt, REAL *out9,   \

@@ -83,7 +83,9 @@ def _structs(suf):
         _fields_ = [("raster_count", P), ("raster_ids", P), ("raster_cap", C.c_int64),
                     ("raster_len", C.c_int64), ("n_rec", C.c_int32), ("rec_ids", P),
                     ("rec_v", P), ("fire_count", P), ("final_v", P), ("final_ie", P),
-                    ("final_ii", P), ("final_refrac", P), ("events", C.c_int64)]
+                    ("final_ii", P), ("final_refrac", P), ("events", C.c_int64),
+                    ("time_from", C.c_int32), ("wall_s", C.c_double),
+                    ("events_from", C.c_int64)]
 
     del real
     _STRUCTS[suf] = (Net, Out)
@@ -135,6 +137,8 @@ class OracleResult:
     final_ii: np.ndarray
     final_refrac: np.ndarray
     events: int
+    wall_s: float = 0.0        # seconds spent in cycles >= time_from
+    events_from: int = 0       # deliveries in cycles >= time_from
 
     def raster(self):
         """List of per-cycle ascending fired-id arrays."""
@@ -143,7 +147,7 @@ class OracleResult:
 
 
 def run(net, dt, steps, rec_ids=(), dtype=np.float32, mode="exact", tile=4096, threads=1,
-        raster_cap=None):
+        raster_cap=None, time_from=-1):
     """Run the oracle for ``steps`` cycles.  mode: "exact" (creation-order
     delivery) or "csr" (tile/OpenMP form, needs the ordering property)."""
     L = lib()
@@ -165,7 +169,7 @@ def run(net, dt, steps, rec_ids=(), dtype=np.float32, mode="exact", tile=4096, t
                   len(rec), _ptr(rec) if len(rec) else None,
                   _ptr(res["rec_v"]) if len(rec) else None, _ptr(res["fire_count"]),
                   _ptr(res["final_v"]), _ptr(res["final_ie"]), _ptr(res["final_ii"]),
-                  _ptr(res["final_refrac"]), 0)
+                  _ptr(res["final_refrac"]), 0, int(time_from), 0.0, 0)
         dt_c = float(np.asarray(dt, a.dtype))
         if mode == "exact":
             rc = getattr(L, f"bso_run_{suf}")(C.byref(a.c), dt_c, steps, C.byref(out))
@@ -180,7 +184,8 @@ def run(net, dt, steps, rec_ids=(), dtype=np.float32, mode="exact", tile=4096, t
             raise RuntimeError(f"oracle run failed: rc={rc}")
         break
     res["raster_ids"] = res["raster_ids"][:out.raster_len].copy()
-    return OracleResult(events=int(out.events), **res)
+    return OracleResult(events=int(out.events), wall_s=float(out.wall_s),
+                        events_from=int(out.events_from), **res)
 
 
 def lif_constants(net, dt, dtype=np.float32):

@@ -15,6 +15,15 @@
 
 #include "bsim_oracle.h"
 
+#include <time.h>
+
+static double bso_now(void)
+{
+    struct timespec ts;
+    clock_gettime(CLOCK_MONOTONIC, &ts);
+    return (double)ts.tv_sec + 1e-9 * (double)ts.tv_nsec;
+}
+
 /* ---- small utilities ----------------------------------------------------- */
 
 typedef struct bso_vec {
