@@ -102,17 +102,22 @@ def host_info():
 
 
 def make_workload(name):
+    """(host network, description): built by the library's seeded generator."""
     if name == "c1":
-        return netspec.c1(), "standard feed-forward ~100 Hz: 64 x 10,000 LIF_brian, ~1,000 " \
-                             "syn/neuron, delays 1..6, dt 0.1 ms"
+        return api.Network.standard(**netspec.C1), \
+            "standard feed-forward ~100 Hz: 64 x 10,000 LIF_brian, ~1,000 syn/neuron, " \
+            "delays 1..6, dt 0.1 ms"
     if name == "c2":
-        return netspec.c2(), "high-activity stress: 100 x 10,000 LIF_brian, ~2,000 syn/neuron, " \
-                             "delays 1..50, dt 0.1 ms"
+        return api.Network.standard(**netspec.C2), \
+            "high-activity stress: 100 x 10,000 LIF_brian, ~2,000 syn/neuron, delays 1..50, " \
+            "dt 0.1 ms"
     if name == "c0":
-        return netspec.c0(), "plain method: 2 x 1,000 LIF_brian, 1e6 all-to-all synapses"
+        return api.Network.from_spec(netspec.c0()), \
+            "plain method: 2 x 1,000 LIF_brian, 1e6 all-to-all synapses"
     if name.startswith("c1x"):   # c1x<npop>: the C1 generator with fewer populations
         k = int(name[3:])
-        return netspec.c1(npop=k), f"standard feed-forward, {k} x 10,000 LIF_brian"
+        return api.Network.standard(**{**netspec.C1, "npop": k}), \
+            f"standard feed-forward, {k} x 10,000 LIF_brian"
     raise SystemExit(f"unknown workload {name}")
 
 
@@ -142,12 +147,13 @@ def main():
         sys.exit(2)
 
     t0 = time.time()
-    spec, desc = make_workload(a.workload)
+    net, desc = make_workload(a.workload)
+    n_neurons, n_syn = net.n, net.s
+    spec = None if a.no_cpu else net.export_spec()
     t_gen = time.time() - t0
-    print(f"[bench] workload {a.workload}: n={spec.n} s={spec.s} (generated in {t_gen:.1f}s)",
+    print(f"[bench] workload {a.workload}: n={n_neurons} s={n_syn} (generated in {t_gen:.1f}s)",
           file=sys.stderr, flush=True)
     t0 = time.time()
-    net = api.Network.from_spec(spec)
     sim = api.Simulation(net, netspec.DT, tile=a.tile)
     del net
     t_build = time.time() - t0
@@ -213,8 +219,8 @@ def main():
         "vs_baseline": None,
         "dtype": "f32",
         "data": "synthetic (seeded generator)",
-        "config": {"workload": desc, "name": a.workload, "neurons": spec.n,
-                   "synapses": spec.s, "dt_s": netspec.DT, "tile": info["tile"],
+        "config": {"workload": desc, "name": a.workload, "neurons": n_neurons,
+                   "synapses": n_syn, "dt_s": netspec.DT, "tile": info["tile"],
                    "simulated_s": a.steps * netspec.DT, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,

@@ -51,6 +51,9 @@ _SIGS = {
     "bsim_sim_get_info": (C.c_int32, [P, P]),
     "bsim_sim_stream": (P, [P]),
     "bsim_sim_destroy": (None, [P]),
+    "bsim_net_gen_standard": (C.c_int64, [P, P]),
+    "bsim_net_export_neurons": (C.c_int32, [P, P, P, P, P]),
+    "bsim_net_export_synapses": (C.c_int64, [P, C.c_int64, C.c_int64, P, P, P, P, P]),
     "bsim_net_lif_constants": (C.c_int32, [P, C.c_float, P, P]),
     "bsim_net_layout": (C.c_int32, [P, C.c_float, C.c_int32, P, P, P, P, P, P]),
 }
@@ -93,6 +96,16 @@ class SimInfo(C.Structure):
     _fields_ = [("n", C.c_int32), ("tile", C.c_int32), ("ntiles", C.c_int32),
                 ("dmax", C.c_int32), ("s", C.c_int64), ("device_bytes", C.c_int64),
                 ("n_src_entries", C.c_int64), ("off_entries", C.c_int64)]
+
+
+class GenParams(C.Structure):
+    _fields_ = [("npop", C.c_int32), ("n", C.c_int32), ("fanout", C.c_int32),
+                ("w0", C.c_float), ("inh_frac", C.c_float), ("dmax", C.c_int32),
+                ("remote_frac", C.c_float), ("drive_rate_hz", C.c_float),
+                ("n_drive", C.c_int32), ("drive_w", C.c_float), ("drive_fanout", C.c_int32),
+                ("seed", C.c_uint64), ("tau_refrac", C.c_float), ("ioff_lo", C.c_float),
+                ("ioff_hi", C.c_float), ("tau_syn_E", C.c_float), ("tau_syn_I", C.c_float),
+                ("v_thresh", C.c_float), ("dt", C.c_float)]
 
 
 class Network:
@@ -185,6 +198,46 @@ class Network:
                         None if spec.tau is None or not np.any(spec.tau) else
                         np.asarray(spec.tau[a:b], np.float32))
         return net
+
+    @classmethod
+    def standard(cls, npop=4, n=1000, fanout=100, w0=1.0, inh_frac=0.25, dmax=6,
+                 remote_frac=0.2, drive_rate_hz=0.0, n_drive=None, drive_w=None,
+                 drive_fanout=None, seed=1, tau_refrac=2e-3, i_offset=(0.0, 0.0),
+                 tau_syn_E=5e-3, tau_syn_I=10e-3, v_thresh=15.0, dt=netspec.DT):
+        """The standard feed-forward workload built by the library's generator
+        (same parameters and same network as netspec.standard)."""
+        net = cls()
+        p = GenParams(npop, n, fanout, w0, inh_frac, dmax, remote_frac, drive_rate_hz,
+                      n if n_drive is None else n_drive,
+                      0.7 * float(np.float32(w0)) if drive_w is None else drive_w,
+                      fanout if drive_fanout is None else drive_fanout, seed, tau_refrac,
+                      i_offset[0], i_offset[1], tau_syn_E, tau_syn_I, v_thresh, dt)
+        r = lib().bsim_net_gen_standard(net._h, C.byref(p))
+        if r < 0:
+            _err(r, "bsim_net_gen_standard")
+        return net
+
+    def export_spec(self) -> netspec.NetSpec:
+        """The network's arrays (creation order) as a NetSpec."""
+        n, s = self.n, self.s
+        kind = np.zeros(n, np.int8)
+        lif = np.zeros((n, netspec.LIF_NPARAM), np.float32)
+        bern = np.zeros(n, np.uint32)
+        seed = C.c_uint64()
+        r = lib().bsim_net_export_neurons(self._h, _p(kind), _p(lif), _p(bern), C.byref(seed))
+        if r < 0:
+            _err(r, "bsim_net_export_neurons")
+        pre, post = np.zeros(s, np.int32), np.zeros(s, np.int32)
+        w, d = np.zeros(s, np.float32), np.zeros(s, np.float32)
+        st = np.zeros(s, np.int8)
+        r = lib().bsim_net_export_synapses(self._h, 0, s, _p(pre), _p(post), _p(w), _p(d),
+                                           _p(st))
+        if r < 0:
+            _err(r, "bsim_net_export_synapses")
+        if np.any(kind == netspec.ARRAY):
+            raise BsimError("export_spec does not carry array-source spike lists")
+        return netspec.NetSpec(kind=kind, lif=lif.astype(np.float64), pre=pre, post=post, w=w,
+                               d=d, stype=st, tau=None, bern_p=bern, seed=int(seed.value))
 
     def lif_constants(self, dt):
         n = self.n

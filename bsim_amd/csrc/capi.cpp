@@ -12,22 +12,19 @@
 
 #include "../../include/bsim_amd.h"
 #include "layout.h"
+#include "internal.h"
 #include "network.h"
 
 extern "C" hipError_t bsim_launch_step(const bsim_dev *d, int32_t t, int32_t flags,
                                        hipStream_t stream);
 
-static thread_local std::string g_err;
+thread_local std::string g_bsim_err;
 
 static int fail(int code, const std::string &msg)
 {
-    g_err = msg;
+    g_bsim_err = msg;
     return code;
 }
-
-struct bsim_network {
-    bsim::HostNet net;
-};
 
 struct bsim_sim {
     int device = 0;
@@ -46,7 +43,7 @@ extern "C" {
 
 int32_t bsim_abi_version(void) { return BSIM_ABI_VERSION; }
 
-const char *bsim_last_error(void) { return g_err.c_str(); }
+const char *bsim_last_error(void) { return g_bsim_err.c_str(); }
 
 /* ---- network construction ------------------------------------------------ */
 
@@ -55,7 +52,7 @@ bsim_network *bsim_net_create(void)
     try {
         return new bsim_network();
     } catch (...) {
-        g_err = "out of host memory";
+        g_bsim_err = "out of host memory";
         return nullptr;
     }
 }

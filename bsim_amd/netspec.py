@@ -185,99 +185,139 @@ def c0(seed: int = 1234, n: int = 1000, tau_syn_E=5e-3, tau_syn_I=10e-3) -> NetS
     return b.build()
 
 
+# ---- counter-based hash shared with the C++ generator (csrc/gen.cpp) ---------
+
+_U64 = np.uint64
+
+
+def _mix64(z):
+    with np.errstate(over="ignore"):
+        z = z + _U64(0x9E3779B97F4A7C15)
+        z = (z ^ (z >> _U64(30))) * _U64(0xBF58476D1CE4E5B9)
+        z = (z ^ (z >> _U64(27))) * _U64(0x94D049BB133111EB)
+    return z ^ (z >> _U64(31))
+
+
+def _h3(seed, stream, counter):
+    with np.errstate(over="ignore"):
+        a = _mix64(_U64(seed) ^ (_U64(stream) * _U64(0xD1B54A32D192ED03)))
+    return _mix64(a ^ np.asarray(counter, _U64))
+
+
+def _u01(x):
+    return (np.asarray(x, _U64) >> _U64(11)).astype(np.float64) * (1.0 / 9007199254740992.0)
+
+
+TAG_TGT, TAG_INH, TAG_W, TAG_D, TAG_IOFF, TAG_REM, TAG_REMDST = 1, 2, 3, 4, 5, 6, 7
+
+
 def standard(npop=4, n=1000, fanout=100, w0=1.0, inh_frac=0.25, dmax=6, remote_frac=0.2,
              drive_rate_hz=0.0, n_drive=None, drive_w=None, drive_fanout=None, seed=1,
              tau_refrac=2e-3, i_offset=(0.0, 0.0), tau_syn_E=5e-3, tau_syn_I=10e-3,
              v_thresh=15.0):
-    """Feed-forward chain of ``npop`` LIF populations of ``n`` neurons.
+    """Feed-forward chain of ``npop`` LIF populations of ``n`` neurons; numpy
+    restatement of the library's generator (bsim_net_gen_standard,
+    csrc/gen.cpp), used for small networks and to pin that generator.
 
     Population i projects to i+1; a ``remote_frac`` share of populations also
     project to one uniformly chosen remote population (not i, not i+1) and
     split their ``fanout`` out-synapses per neuron evenly between the two
-    projections.  Targets of a pre neuron within a projection are distinct,
-    drawn by stratified sampling (one target in each of f equal strata).
+    projections.  Within a projection, pre neuron i draws one target in each
+    of f disjoint integer strata of the destination (distinct targets).
     Excitatory weights ~ U[0, 0.7 w0), a share ``inh_frac`` inhibitory with
     weights ~ U[-0.5 w0, 0); integer delays U{1..dmax} steps.  LIF neurons:
     tau_m 20 ms, R = 1, v_rest = v_reset = 0, i_offset ~ U[i_offset).
     Population 0 is driven by ``n_drive`` Bernoulli sources firing at
-    ``drive_rate_hz`` (projection drive -> pop 0, ``drive_fanout`` targets
-    each, weight ``drive_w``).  Synapses are created source population by
-    source population, pre-major, so every target's inputs arrive in
-    ascending (pre, delay) creation order.
+    ``drive_rate_hz`` (``drive_fanout`` targets each, weight ``drive_w``).
+    All draws come from a counter-based hash of ``seed``.  Synapses are
+    created source population by source population, pre-major, so every
+    target's inputs are created in ascending pre order.
     """
-    rng = np.random.Generator(np.random.PCG64(seed))
-    tau_m = 20e-3
+    f32 = lambda x: float(np.float32(x))  # noqa: E731  (the C struct holds floats)
+    w0, inh_frac, remote_frac = f32(w0), f32(inh_frac), f32(remote_frac)
+    drive_rate_hz, tau_refrac, dtf = f32(drive_rate_hz), f32(tau_refrac), f32(DT)
+    if n_drive is None:
+        n_drive = n
+    if drive_w is None:
+        drive_w = 0.7 * w0
+    drive_w = f32(drive_w)
+    if drive_fanout is None:
+        drive_fanout = fanout
+    lo_i, hi_i = f32(i_offset[0]), f32(i_offset[1])
     kinds, rows_l, bern_l, pops = [], [], [], []
     nn = 0
-    drive = None
-    if drive_rate_hz > 0:
-        nd = n if n_drive is None else n_drive
-        p = min(1.0, drive_rate_hz * DT)
-        thr = np.uint32(min(0xFFFFFFFF, int(round(p * 4294967296.0))))
-        kinds.append(np.full(nd, BERNOULLI, np.int8))
-        rows_l.append(np.zeros((nd, LIF_NPARAM)))
-        bern_l.append(np.full(nd, thr, np.uint32))
-        drive = 0
-        pops.append((0, nd))
-        nn = nd
-    first_lif = len(pops)
-    for _ in range(npop):
-        rows = np.tile(lif_row(0.0, 0.0, 0.0, tau_m, tau_m, tau_refrac, tau_syn_E, tau_syn_I,
-                               v_thresh, 0.0), (n, 1))
-        if i_offset[1] > i_offset[0]:
-            rows[:, 9] = rng.uniform(i_offset[0], i_offset[1], n).astype(np.float32)
-        kinds.append(np.full(n, LIF, np.int8))
+    drive = drive_rate_hz > 0 and n_drive > 0
+    if drive:
+        pr = min(1.0, drive_rate_hz * dtf)
+        thr = np.uint32(min(4294967295.0, np.round(pr * 4294967296.0)))
+        kinds.append(np.full(n_drive, BERNOULLI, np.int8))
+        rows = np.zeros((n_drive, LIF_NPARAM))
+        rows[:, 10:12] = f32(1e-3)
         rows_l.append(rows)
-        bern_l.append(np.zeros(n, np.uint32))
-        pops.append((nn, n))
-        nn += n
-    remotes = {}
+        bern_l.append(np.full(n_drive, thr, np.uint32))
+        pops.append((0, n_drive))
+        nn = n_drive
+    first_lif = len(pops)
+    g = np.arange(nn, nn + npop * n, dtype=np.uint64)
+    ioff = (np.zeros(len(g), np.float32) if not hi_i > lo_i else
+            (lo_i + (hi_i - lo_i) * _u01(_h3(seed, TAG_IOFF << 32, g))).astype(np.float32))
+    rows = np.tile(lif_row(0.0, 0.0, 0.0, 20e-3, 20e-3, tau_refrac, tau_syn_E, tau_syn_I,
+                           v_thresh, 0.0), (npop * n, 1)).astype(np.float32).astype(np.float64)
+    rows[:, 9] = ioff
+    kinds.append(np.full(npop * n, LIF, np.int8))
+    rows_l.append(rows)
+    bern_l.append(np.zeros(npop * n, np.uint32))
     for i in range(npop):
-        if rng.random() < remote_frac and npop > 2:
-            choices = [j for j in range(npop) if j != i and j != i + 1]
-            remotes[i] = int(rng.choice(choices))
-    proj = []   # (src pop index, dst pop index, fan-out)
-    if drive is not None:
-        proj.append((drive, first_lif, min(fanout if drive_fanout is None else drive_fanout, n)))
+        pops.append((nn + i * n, n))
+    remote = [-1] * npop
+    for i in range(npop):
+        if npop > 2 and _u01(_h3(seed, TAG_REM << 32, i)) < remote_frac:
+            nch = npop - (2 if i + 1 < npop else 1)
+            c = min(int(_u01(_h3(seed, TAG_REMDST << 32, i)) * nch), nch - 1)
+            remote[i] = [j for j in range(npop) if j != i and j != i + 1][c]
+    proj = []
+    if drive:
+        proj.append((0, first_lif, min(drive_fanout, n)))
     for i in range(npop):
         src = first_lif + i
         if i + 1 < npop:
-            f = fanout - fanout // 2 if i in remotes else fanout
-            proj.append((src, src + 1, min(f, n)))
-        if i in remotes:
-            proj.append((src, first_lif + remotes[i], min(fanout // 2, n)))
-    proj.sort(key=lambda x: x[0])  # creation: ascending source population
+            proj.append((src, src + 1, min(fanout - fanout // 2 if remote[i] >= 0 else fanout, n)))
+        if remote[i] >= 0:
+            proj.append((src, first_lif + remote[i], min(fanout // 2, n)))
     total = sum(pops[a][1] * f for a, _, f in proj)
     pre = np.empty(total, np.int32)
     post = np.empty(total, np.int32)
     w = np.empty(total, np.float32)
     d = np.empty(total, np.float32)
-    st = np.empty(total, np.int8)
+    st = np.zeros(total, np.int8)
     o = 0
-    for src, dst, f in proj:
+    for pi, (src, dst, f) in enumerate(proj):
         s0, sn = pops[src]
         d0, dn = pops[dst]
         m = sn * f
-        strata = dn / f
+        c = np.arange(m, dtype=np.uint64)
+        k = np.tile(np.arange(f, dtype=np.int64), sn)
+        lo = (k * dn) // f
+        hi = ((k + 1) * dn) // f
+        off = lo + (_u01(_h3(seed, (TAG_TGT << 32) | pi, c)) * (hi - lo).astype(np.float64)
+                    ).astype(np.int64)
+        off = np.minimum(off, hi - 1)
         pre[o:o + m] = np.repeat(np.arange(s0, s0 + sn, dtype=np.int32), f)
-        base = (np.arange(f) * strata)[None, :]
-        off = np.floor(base + rng.random((sn, f)) * strata).astype(np.int64)
-        post[o:o + m] = d0 + np.minimum(off, dn - 1).reshape(-1)
-        if src == drive:
-            w[o:o + m] = np.float32(0.7 * w0 if drive_w is None else drive_w)
-            st[o:o + m] = 0
+        post[o:o + m] = d0 + off
+        if drive and src == 0:
+            w[o:o + m] = np.float32(drive_w)
         else:
-            inh = rng.random(m) < inh_frac
-            w[o:o + m] = np.where(inh, rng.uniform(-0.5 * w0, 0.0, m),
-                                  rng.uniform(0.0, 0.7 * w0, m)).astype(np.float32)
+            inh = _u01(_h3(seed, (TAG_INH << 32) | pi, c)) < inh_frac
+            u = _u01(_h3(seed, (TAG_W << 32) | pi, c))
+            w[o:o + m] = np.where(inh, -0.5 * w0 * u, 0.7 * w0 * u).astype(np.float32)
             st[o:o + m] = inh
-        k = rng.integers(1, dmax + 1, m)
-        d[o:o + m] = k.astype(np.float32) * np.float32(DT)
+        dk = np.minimum(1 + (_u01(_h3(seed, (TAG_D << 32) | pi, c)) * dmax).astype(np.int64),
+                        dmax)
+        d[o:o + m] = dk.astype(np.float32) * np.float32(dtf)
         o += m
-    spec = NetSpec(kind=np.concatenate(kinds), lif=np.concatenate(rows_l), pre=pre, post=post,
+    return NetSpec(kind=np.concatenate(kinds), lif=np.concatenate(rows_l), pre=pre, post=post,
                    w=w, d=d, stype=st, tau=None, bern_p=np.concatenate(bern_l), seed=seed,
                    pops=pops)
-    return spec
 
 
 # The standard feed-forward workload at ~100 Hz: 64 x 10,000 LIF, ~1,000

@@ -159,6 +159,40 @@ void *bsim_sim_stream(bsim_sim *sim);
 
 void bsim_sim_destroy(bsim_sim *sim);
 
+/* ---- workload generation and export (host) -------------------------------- */
+
+/* Parameters of the standard feed-forward workload (see bsim_net_gen_standard). */
+typedef struct bsim_gen_params {
+    int32_t npop, n, fanout;      /* populations, neurons each, out-synapses per neuron */
+    float w0, inh_frac;           /* exc. w ~ U[0, 0.7 w0), inh. w ~ U[-0.5 w0, 0) */
+    int32_t dmax;                 /* delays U{1..dmax} steps */
+    float remote_frac;            /* share of populations with a remote projection */
+    float drive_rate_hz;          /* Bernoulli drive of population 0 (0 = none) */
+    int32_t n_drive;
+    float drive_w;
+    int32_t drive_fanout;
+    uint64_t seed;
+    float tau_refrac, ioff_lo, ioff_hi, tau_syn_E, tau_syn_I, v_thresh;
+    float dt;
+} bsim_gen_params;
+
+/* Builds the standard feed-forward chain into an empty network: optional
+ * drive sources, then npop LIF_brian populations (tau_m 20 ms, R 1,
+ * v_rest = v_reset = 0, i_offset ~ U[ioff_lo, ioff_hi)); population i
+ * projects to i+1 and, for a remote_frac share, to one remote population,
+ * splitting its fan-out; targets by stratified sampling; everything drawn
+ * from a counter-based hash of the seed.  The reference's benchmark program
+ * builds the same topology from all-to-all projections
+ * (test/gpu/standard_test.cpp:92-139).  Returns the number of synapses. */
+int64_t bsim_net_gen_standard(bsim_network *net, const bsim_gen_params *p);
+
+/* Copies out the network as stored (creation order); NULL arrays skipped.
+ * lif receives n * 12 floats. */
+int32_t bsim_net_export_neurons(const bsim_network *net, int8_t *kind, float *lif,
+                                uint32_t *bern_p, uint64_t *seed);
+int64_t bsim_net_export_synapses(const bsim_network *net, int64_t start, int64_t count,
+                                 int32_t *pre, int32_t *post, float *w, float *d, int8_t *stype);
+
 /* ---- host-side inspection (no device needed) ------------------------------ */
 
 /* LIF constants as LIFENeuron::reset computes them: per neuron Cm CE CI C_E

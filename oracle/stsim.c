@@ -105,6 +105,39 @@ static void bso_sort_by_key3(int64_t *v, int64_t n, const int32_t *pre, const in
     free(tmp);
 }
 
+/* Sorts the synapse indices of one tile (given in creation order) by
+ * (pre, delay_steps, creation): keys pre << 32 | delay << 24 | rank, LSD
+ * radix sort with 16-bit digits.  Returns -1 if the tile is too large. */
+static int bso_sort_tile(int64_t *v, int64_t n, const int32_t *pre, const int32_t *ds)
+{
+    if (n < 2) return 0;
+    if (n >= (1 << 24)) {
+        bso_sort_by_key3(v, n, pre, ds);
+        return 0;
+    }
+    uint64_t *a = (uint64_t *)malloc((size_t)n * sizeof(uint64_t));
+    uint64_t *b = (uint64_t *)malloc((size_t)n * sizeof(uint64_t));
+    int64_t *c = (int64_t *)malloc(65537 * sizeof(int64_t));
+    if (!a || !b || !c) {
+        free(a); free(b); free(c);
+        return -1;
+    }
+    for (int64_t i = 0; i < n; i++)
+        a[i] = ((uint64_t)(uint32_t)pre[v[i]] << 32) | ((uint64_t)(ds[v[i]] & 0xFF) << 24) | (uint64_t)i;
+    for (int shift = 0; shift < 64; shift += 16) {
+        memset(c, 0, 65537 * sizeof(int64_t));
+        for (int64_t i = 0; i < n; i++) c[((a[i] >> shift) & 0xFFFF) + 1]++;
+        for (int j = 0; j < 65536; j++) c[j + 1] += c[j];
+        for (int64_t i = 0; i < n; i++) b[c[(a[i] >> shift) & 0xFFFF]++] = a[i];
+        uint64_t *t = a; a = b; b = t;
+    }
+    int64_t *o = (int64_t *)b;  /* reuse: 8-byte slots */
+    for (int64_t i = 0; i < n; i++) o[i] = v[a[i] & 0xFFFFFF];
+    memcpy(v, o, (size_t)n * sizeof(int64_t));
+    free(a); free(b); free(c);
+    return 0;
+}
+
 uint32_t bso_hash(uint64_t seed, uint32_t id, uint32_t t)
 {
     uint64_t z = seed ^ (((uint64_t)id << 32) | (uint64_t)t);
