@@ -305,9 +305,10 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
     d.tile = L.tile;
     d.ntiles = L.ntiles;
     d.dmax = L.dmax;
+    d.nwords = (L.n + 63) / 64;
     d.seed = L.seed;
     sim->s = L.s;
-    sim->n_src = (int64_t)L.src.size();
+    sim->n_src = (int64_t)L.rng.size();
     sim->n_off = (int64_t)L.off_tab.size();
 #define UP(field, vec)                                   \
     do {                                                 \
@@ -324,8 +325,8 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
     UP(tile_syn, L.tile_syn);
     UP(syn_tgt, L.syn_tgt);
     UP(syn_w, L.syn_w);
-    UP(tile_src, L.tile_src);
-    UP(src, L.src);
+    UP(tile_rng, L.tile_rng);
+    UP(rng, L.rng);
     UP(off_tab, L.off_tab);
     /* state */
     UP(v, L.v_init);
@@ -334,25 +335,23 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
     {
         const size_t n = (size_t)(L.n > 0 ? L.n : 1);
         const size_t nt = (size_t)(L.ntiles > 0 ? L.ntiles : 1);
+        const size_t nw = (size_t)(d.nwords > 0 ? d.nwords : 1);
         if (!(d.ie = dev_alloc<float>(sim, n, e))) goto hip_fail;
         if (!(d.ii = dev_alloc<float>(sim, n, e))) goto hip_fail;
         if (!(d.refrac = dev_alloc<int32_t>(sim, n, e))) goto hip_fail;
-        if (!(d.hist = dev_alloc<uint64_t>(sim, n, e))) goto hip_fail;
         if (!(d.fire_count = dev_alloc<int32_t>(sim, n, e))) goto hip_fail;
-        if (!(d.act_cnt = dev_alloc<int32_t>(sim, 2 * nt, e))) goto hip_fail;
-        if (!(d.act_ids = dev_alloc<int32_t>(sim, 2 * n, e))) goto hip_fail;
-        if (!(d.act_hist = dev_alloc<uint64_t>(sim, 2 * n, e))) goto hip_fail;
+        if (!(d.hist = dev_alloc<uint64_t>(sim, 2 * n, e))) goto hip_fail;
+        if (!(d.active = dev_alloc<uint64_t>(sim, 2 * nw, e))) goto hip_fail;
         if (!(d.ev_count = dev_alloc<int64_t>(sim, nt, e))) goto hip_fail;
         if (!(sim->ev_fin = dev_alloc<int64_t>(sim, nt, e))) goto hip_fail;
         hipMemsetAsync(d.ie, 0, n * 4, sim->stream);
         hipMemsetAsync(d.ii, 0, n * 4, sim->stream);
         hipMemsetAsync(d.refrac, 0, n * 4, sim->stream);
-        hipMemsetAsync(d.hist, 0, n * 8, sim->stream);
         hipMemsetAsync(d.fire_count, 0, n * 4, sim->stream);
-        hipMemsetAsync(d.act_cnt, 0, 2 * nt * 4, sim->stream);
+        hipMemsetAsync(d.hist, 0, 2 * n * 8, sim->stream);
+        hipMemsetAsync(d.active, 0, 2 * nw * 8, sim->stream);
         hipMemsetAsync(d.ev_count, 0, nt * 8, sim->stream);
         hipMemsetAsync(sim->ev_fin, 0, nt * 8, sim->stream);
-        d.nwords = (L.n + 63) / 64;
         if (c.rec_steps > 0) {
             d.rec_steps = c.rec_steps;
             d.rec_t0 = 0;

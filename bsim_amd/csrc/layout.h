@@ -3,22 +3,26 @@
  *
  * Neurons are numbered in creation order (the order BSim's CPU simulator
  * updates them and reports spikes in).  They are cut into tiles of TILE
- * consecutive neurons (TILE a multiple of 64); a tile is the unit of work of
- * one wavefront in the step kernel: it receives the synapses whose target lies
- * in the tile and updates the tile's neurons.
+ * consecutive neurons (TILE a multiple of 256).  A tile is the unit of work
+ * of one 256-thread workgroup of the step kernel: it receives the synapses
+ * whose target lies in the tile and updates the tile's neurons.  Each of the
+ * workgroup's four wavefronts owns the accumulators of one quarter of the
+ * tile's targets.
  *
  * Synapses are stored tile-major: all synapses into tile k, sorted by
  * (pre, delay_steps, creation index) -- the (pre, delay) CSR of the reference
  * (src/net/BuildNetwork.cpp:96-176) restricted to the tile -- as a 16-bit local
- * target and an fp32 weight.  A tile reads its rows through "source entries":
- * one per source tile s that has synapses into it, covering pres [lo, hi) of
- * s, with an offset table of (hi - lo) * D + 1 uint32 row starts relative to
- * the tile's first synapse (row (p, d) = [off[(p-lo)*D + d-1], next)).
+ * target and an fp32 weight.  The pres with synapses into a tile form a few
+ * contiguous "source ranges" [lo, hi); each has a table of (hi - lo) * D + 1
+ * uint32 row starts relative to the tile's first synapse: row (p, d) is
+ * [off[(p - lo) * D + d - 1], off[(p - lo) * D + d]).
  *
  * Spike history: hist[n] bit j = neuron n fired j cycles ago (bit 0 = this
- * cycle).  After updating cycle t a tile publishes the ascending list of its
- * neurons with any of bits 1..D set, with their history word: exactly the
- * pres whose (pre, d) rows deliver in phase (ii) of cycle t (spike at t - d).
+ * cycle).  After updating cycle t, a tile writes its neurons' history words
+ * and one bit per neuron in the "active" bitmap: set iff any of bits 1..D of
+ * the history is set, i.e. iff some (n, d) row delivers in phase (ii) of cycle
+ * t.  Both are double-buffered by cycle parity, so that a launch reads the
+ * previous cycle's copy while it writes the current one.
  */
 #ifndef BSIM_LAYOUT_H
 #define BSIM_LAYOUT_H
@@ -26,16 +30,15 @@
 #include <stdint.h>
 
 #define BSIM_MAX_DELAY 63   /* history word: bits 1..D of a uint64 */
-#define BSIM_MAX_TILE 3840  /* LDS: 16 bytes per target + 2.5 KiB, within 64 KiB */
+#define BSIM_TILE_ALIGN 256 /* tiles are whole multiples of the workgroup */
+#define BSIM_MAX_TILE 2048  /* 16 bytes of LDS per target, see bsim_step_lds_bytes */
 
 enum { BSIM_KIND_LIF = 0, BSIM_KIND_ARRAY = 1, BSIM_KIND_BERNOULLI = 2 };
 
-typedef struct bsim_src_entry {
-    int32_t src_tile;   /* tile whose active list is read */
-    int32_t lo, hi;     /* pre range (global ids) within that tile */
-    int32_t pad;
+typedef struct bsim_range {
+    int32_t lo, hi;     /* pre neurons [lo, hi) with synapses into the tile */
     int64_t off_base;   /* first index into off_tab */
-} bsim_src_entry;
+} bsim_range;
 
 /* Kernel arguments (passed by value). */
 typedef struct bsim_dev {
@@ -43,6 +46,8 @@ typedef struct bsim_dev {
     int32_t tile;        /* TILE */
     int32_t ntiles;
     int32_t dmax;        /* D, 1..BSIM_MAX_DELAY */
+    int32_t nwords;      /* (n + 63) / 64 */
+    int32_t pad0;
     /* neuron constants (per neuron) */
     const int8_t *kind;
     const float *Cm, *CE, *CI, *C_E, *C_I, *v_tmp, *v_thresh, *v_reset;
@@ -55,25 +60,23 @@ typedef struct bsim_dev {
     float *v, *ie, *ii;
     int32_t *refrac;
     int32_t *arr_cur;
-    uint64_t *hist;
     int32_t *fire_count;
+    uint64_t *hist;             /* [2][n] by cycle parity */
+    uint64_t *active;           /* [2][nwords] by cycle parity */
     /* synapses */
     const int64_t *tile_syn;    /* [ntiles + 1] first synapse of each tile */
     const uint16_t *syn_tgt;
     const float *syn_w;
-    const int32_t *tile_src;    /* [ntiles + 1] first source entry of each tile */
-    const bsim_src_entry *src;
+    const int32_t *tile_rng;    /* [ntiles + 1] first source range of each tile */
+    const bsim_range *rng;
     const uint32_t *off_tab;
-    /* active lists, double-buffered by cycle parity: [2][ntiles], [2][n] */
-    int32_t *act_cnt;
-    int32_t *act_ids;
-    uint64_t *act_hist;
     /* accounting / recording */
     int64_t *ev_count;          /* [ntiles] deliveries performed */
     uint64_t *raster;           /* [rec_steps][nwords] fired bitmaps, or NULL */
-    int32_t rec_steps, nwords;
+    int32_t rec_steps;
     int32_t rec_t0;             /* cycle of raster/trace row 0 */
     int32_t n_rec;
+    int32_t pad1;
     const int32_t *rec_index;   /* [n]: trace column of a neuron or -1 (NULL: none) */
     float *rec_v;               /* [rec_steps][n_rec] */
 } bsim_dev;

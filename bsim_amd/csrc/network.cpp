@@ -16,6 +16,10 @@ namespace bsim {
 
 static const float kZero = 1e-10f;  // `const real ZERO = 1e-10;` (src/utils/constant.h:19)
 
+/* Two pres with synapses into a tile share a source range when fewer than
+ * this many pres without synapses into it lie between them. */
+static const int32_t kRangeGap = 256;
+
 void lif_constants(const float *p, float dt, float out[9], int32_t *refrac_time)
 {
     const float v_init = p[0], v_rest = p[1], v_reset = p[2], cm = p[3];
@@ -64,11 +68,12 @@ bool synapse_reset(float w, float delay, float tau, float dt, float *w_out, int3
 
 int32_t default_tile(int32_t n, int32_t cus)
 {
+    /* two workgroups (tiles) per compute unit */
     if (cus <= 0) cus = 256;
-    int64_t want = ((int64_t)n + (int64_t)cus * 4 - 1) / ((int64_t)cus * 4);
-    int64_t t = ((want + 63) / 64) * 64;
-    if (t < 64) t = 64;
-    if (t > 2048) t = 2048;
+    int64_t want = ((int64_t)n + (int64_t)cus * 2 - 1) / ((int64_t)cus * 2);
+    int64_t t = ((want + BSIM_TILE_ALIGN - 1) / BSIM_TILE_ALIGN) * BSIM_TILE_ALIGN;
+    if (t < BSIM_TILE_ALIGN) t = BSIM_TILE_ALIGN;
+    if (t > BSIM_MAX_TILE) t = BSIM_MAX_TILE;
     return (int32_t)t;
 }
 
@@ -76,8 +81,9 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
 {
     const int32_t N = net.n();
     const int64_t S = net.s();
-    if (tile <= 0 || tile % 64 != 0 || tile > BSIM_MAX_TILE) {
-        err = "tile size must be a positive multiple of 64 up to " + std::to_string(BSIM_MAX_TILE);
+    if (tile <= 0 || tile % BSIM_TILE_ALIGN != 0 || tile > BSIM_MAX_TILE) {
+        err = "tile size must be a positive multiple of " + std::to_string(BSIM_TILE_ALIGN) +
+              " up to " + std::to_string(BSIM_MAX_TILE);
         return -1;
     }
     if (tile > 65536) {
@@ -193,8 +199,8 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
     }
     L.syn_tgt.resize(S);
     L.syn_w.resize(S);
-    L.tile_src.assign((size_t)NT + 1, 0);
-    std::vector<std::vector<bsim_src_entry>> tsrc(NT);
+    L.tile_rng.assign((size_t)NT + 1, 0);
+    std::vector<std::vector<bsim_range>> trng(NT);
     std::vector<std::vector<uint32_t>> toff(NT);
     const int32_t D = dmax;
     int fail = 0;
@@ -214,30 +220,29 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
             L.syn_tgt[lo + e] = (uint16_t)(net.post[k] - t * tile);
             L.syn_w[lo + e] = w[k];
         }
-        /* source entries: one per source tile present, pre range clipped */
+        /* source ranges: runs of pres whose gaps are at most kRangeGap */
         int64_t e = 0;
         while (e < m) {
             const int32_t p0 = (int32_t)(key[e] >> 32);
-            const int32_t st = p0 / tile;
             int64_t f = e;
             int32_t plast = p0;
-            while (f < m && (int32_t)(key[f] >> 32) / tile == st) {
-                plast = (int32_t)(key[f] >> 32);
+            while (f < m) {
+                const int32_t pf = (int32_t)(key[f] >> 32);
+                if (pf - plast > kRangeGap) break;
+                plast = pf;
                 f++;
             }
-            bsim_src_entry se;
-            se.src_tile = st;
-            se.lo = p0;
-            se.hi = plast + 1;
-            se.pad = 0;
-            se.off_base = (int64_t)toff[t].size();
-            const int64_t cnt = (int64_t)(se.hi - se.lo) * D + 1;
+            bsim_range r;
+            r.lo = p0;
+            r.hi = plast + 1;
+            r.off_base = (int64_t)toff[t].size();
+            const int64_t cnt = (int64_t)(r.hi - r.lo) * D + 1;
             const size_t base = toff[t].size();
             toff[t].resize(base + cnt);
             int64_t pos = e;
-            for (int32_t p = se.lo; p < se.hi; p++) {
+            for (int32_t p = r.lo; p < r.hi; p++) {
                 for (int32_t d = 1; d <= D; d++) {
-                    toff[t][base + (size_t)(p - se.lo) * D + (d - 1)] = (uint32_t)pos;
+                    toff[t][base + (size_t)(p - r.lo) * D + (d - 1)] = (uint32_t)pos;
                     while (pos < f && (int32_t)(key[pos] >> 32) == p &&
                            (int32_t)((key[pos] >> 26) & 63) == d)
                         pos++;
@@ -245,7 +250,7 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
             }
             toff[t][base + cnt - 1] = (uint32_t)pos;
             if (pos != f) fail = 1;
-            tsrc[t].push_back(se);
+            trng[t].push_back(r);
             e = f;
         }
     }
@@ -253,22 +258,22 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
         err = "internal error: CSR row walk mismatch";
         return -7;
     }
-    size_t nsrc = 0, noff = 0;
+    size_t nr = 0, noff = 0;
     for (int32_t t = 0; t < NT; t++) {
-        L.tile_src[t + 1] = L.tile_src[t] + (int32_t)tsrc[t].size();
-        nsrc += tsrc[t].size();
+        L.tile_rng[t + 1] = L.tile_rng[t] + (int32_t)trng[t].size();
+        nr += trng[t].size();
         noff += toff[t].size();
     }
-    L.src.reserve(nsrc);
+    L.rng.reserve(nr);
     L.off_tab.reserve(noff);
     for (int32_t t = 0; t < NT; t++) {
         const int64_t ob = (int64_t)L.off_tab.size();
-        for (auto se : tsrc[t]) {
-            se.off_base += ob;
-            L.src.push_back(se);
+        for (auto r : trng[t]) {
+            r.off_base += ob;
+            L.rng.push_back(r);
         }
         L.off_tab.insert(L.off_tab.end(), toff[t].begin(), toff[t].end());
-        std::vector<bsim_src_entry>().swap(tsrc[t]);
+        std::vector<bsim_range>().swap(trng[t]);
         std::vector<uint32_t>().swap(toff[t]);
     }
     return 0;

@@ -54,8 +54,8 @@ struct Layout {
     std::vector<int64_t> tile_syn;
     std::vector<uint16_t> syn_tgt;
     std::vector<float> syn_w;
-    std::vector<int32_t> tile_src;
-    std::vector<bsim_src_entry> src;
+    std::vector<int32_t> tile_rng;
+    std::vector<bsim_range> rng;
     std::vector<uint32_t> off_tab;
 };
 

@@ -52,7 +52,7 @@ def assert_same(g, o, what):
 GOLDEN_F32 = [n for n in golden_io.case_names() if n.endswith("_f32")]
 
 
-@pytest.mark.parametrize("tile", [0, 64])
+@pytest.mark.parametrize("tile", [0, 256])
 @pytest.mark.parametrize("name", GOLDEN_F32)
 def test_golden_cases(name, tile):
     """GPU == reference CPU build, bit for bit."""
@@ -76,7 +76,7 @@ def test_golden_cases(name, tile):
 
 
 @pytest.mark.parametrize("seed", range(6))
-@pytest.mark.parametrize("tile", [64, 128, 0])
+@pytest.mark.parametrize("tile", [256, 512, 0])
 def test_random_networks_vs_oracle(seed, tile):
     spec = netspec.random_net(100 + seed, n_lif=300, n_arr=20, n_bern=20, p_conn=0.05,
                               dmax=1 + 7 * seed, allow_dup=seed % 2 == 0, spike_prob=0.1)
@@ -93,7 +93,7 @@ def test_max_delay_63():
     spec = netspec.standard(npop=3, n=300, fanout=40, dmax=63, w0=4.0, drive_rate_hz=300.0,
                             n_drive=100, drive_w=14.0, seed=5, i_offset=(12.0, 14.5))
     say(f"[gpu] dmax63 n={spec.n} s={spec.s}")
-    r = run_gpu(spec, 500, [150, 450, 750], 64)
+    r = run_gpu(spec, 500, [150, 450, 750], 256)
     o = oracle.run(spec, DT, 500, rec_ids=[150, 450, 750])
     assert r["info"]["dmax"] == 63
     assert_same(r, o, "dmax63")
@@ -138,3 +138,40 @@ def test_step_in_chunks_equals_one_call():
     s2.sync()
     np.testing.assert_array_equal(s1.raster()[1], s2.raster()[1])
     assert np.array_equal(s1.trace().view(np.uint32), s2.trace().view(np.uint32))
+
+
+def dense_net(n_src=700, n_tgt=300, p_fire=0.9, dmax=3, seed=4):
+    """All-to-all from busy Bernoulli sources, every (pre, post) pair created
+    twice in a row: ~600 active pres per cycle, rows of 2 x 256 events, so
+    the item, segment and boundary-table passes each overflow their staging
+    capacity several times per cycle."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = netspec.Builder()
+    b.population(np.zeros((n_src, netspec.LIF_NPARAM)), kind=netspec.BERNOULLI,
+                 bern_p=np.uint32(int(p_fire * 2 ** 32)))
+    rows = np.tile(netspec.lif_row(0.0, 0.0, 0.0, 0.02, 0.02, 2e-4, 5e-3, 8e-3, 15.0, 0.0),
+                   (n_tgt, 1))
+    rows[:, 9] = rng.uniform(10.0, 14.0, n_tgt)
+    b.population(rows)
+    pre = np.repeat(np.arange(n_src), 2 * n_tgt)
+    post = n_src + np.tile(np.repeat(np.arange(n_tgt), 2), n_src)
+    m = len(pre)
+    w = np.where(rng.random(m) < 0.3, -rng.uniform(0, 0.02, m), rng.uniform(0, 0.01, m))
+    k = rng.integers(1, dmax + 1, n_src)            # one delay per pre: rows stay whole
+    d = np.repeat(k, 2 * n_tgt).astype(np.float32) * np.float32(DT)
+    b.connect(pre, post, w.astype(np.float32), d, (w < 0).astype(np.int8))
+    spec = b.build()
+    spec.seed = seed
+    return spec
+
+
+@pytest.mark.parametrize("tile", [256, 512])
+def test_dense_overflow_paths(tile):
+    spec = dense_net()
+    steps = 60
+    rec = [700, 850, 999]
+    say(f"[gpu] dense n={spec.n} s={spec.s} tile={tile}")
+    r = run_gpu(spec, steps, rec, tile)
+    o = oracle.run(spec, DT, steps, rec_ids=rec, mode="csr", tile=256, threads=8)
+    assert o.events > steps * 100_000
+    assert_same(r, o, "dense")
