@@ -4,18 +4,21 @@
  * Neurons are numbered in creation order (the order BSim's CPU simulator
  * updates them and reports spikes in).  They are cut into tiles of TILE
  * consecutive neurons (TILE a multiple of 256).  A tile is the unit of work
- * of one 256-thread workgroup of the step kernel: it receives the synapses
- * whose target lies in the tile and updates the tile's neurons.  Each of the
- * workgroup's four wavefronts owns the accumulators of one quarter of the
- * tile's targets.
+ * of one workgroup of BSIM_NWAVE wavefronts: it receives the synapses whose
+ * target lies in the tile and updates the tile's neurons.  Wavefront q owns
+ * the targets of quarter q, local targets [q * TILE/4, (q + 1) * TILE/4), and
+ * within it lane l owns the local targets congruent to l modulo 64.
  *
  * Synapses are stored tile-major: all synapses into tile k, sorted by
- * (pre, delay_steps, creation index) -- the (pre, delay) CSR of the reference
- * (src/net/BuildNetwork.cpp:96-176) restricted to the tile -- as a 16-bit local
- * target and an fp32 weight.  The pres with synapses into a tile form a few
- * contiguous "source ranges" [lo, hi); each has a table of (hi - lo) * D + 1
- * uint32 row starts relative to the tile's first synapse: row (p, d) is
- * [off[(p - lo) * D + d - 1], off[(p - lo) * D + d]).
+ * (pre, delay_steps, quarter of the target, creation index) -- the (pre, delay)
+ * CSR of the reference (src/net/BuildNetwork.cpp:96-176) restricted to the
+ * tile, each row split into its four quarters -- as a 16-bit local target and
+ * an fp32 weight.  Splitting a row by quarter keeps, for every target, the
+ * (pre, delay, creation) order of its inputs.  The pres with synapses into a
+ * tile form a few contiguous "source ranges" [lo, hi); each has a table of
+ * (hi - lo) * D * 4 + 1 uint32 starts relative to the tile's first synapse:
+ * sub-row (p, d, q) is [off[e], off[e + 1]) with
+ * e = ((p - lo) * D + d - 1) * 4 + q.
  *
  * Spike history: hist[n] bit j = neuron n fired j cycles ago (bit 0 = this
  * cycle).  After updating cycle t, a tile writes its neurons' history words
@@ -30,8 +33,9 @@
 #include <stdint.h>
 
 #define BSIM_MAX_DELAY 63   /* history word: bits 1..D of a uint64 */
-#define BSIM_TILE_ALIGN 256 /* tiles are whole multiples of the workgroup */
-#define BSIM_MAX_TILE 2048  /* 16 bytes of LDS per target, see bsim_step_lds_bytes */
+#define BSIM_NWAVE 4        /* wavefronts per tile = target quarters */
+#define BSIM_TILE_ALIGN 256 /* BSIM_NWAVE quarters of whole 64-lane wavefronts */
+#define BSIM_MAX_TILE 2048  /* 8 bytes of LDS per target, see bsim_step_lds_bytes */
 
 enum { BSIM_KIND_LIF = 0, BSIM_KIND_ARRAY = 1, BSIM_KIND_BERNOULLI = 2 };
 
@@ -69,7 +73,7 @@ typedef struct bsim_dev {
     const float *syn_w;
     const int32_t *tile_rng;    /* [ntiles + 1] first source range of each tile */
     const bsim_range *rng;
-    const uint32_t *off_tab;
+    const uint32_t *off_tab;    /* BSIM_NWAVE sub-row starts per (pre, delay) */
     /* accounting / recording */
     int64_t *ev_count;          /* [ntiles] deliveries performed */
     uint64_t *raster;           /* [rec_steps][nwords] fired bitmaps, or NULL */

@@ -192,8 +192,8 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
         for (int64_t k = 0; k < S; k++) order[fill[net.post[k] / tile]++] = k;
     }
     for (int32_t t = 0; t < NT; t++) {
-        if (L.tile_syn[t + 1] - L.tile_syn[t] >= ((int64_t)1 << 26)) {
-            err = "a tile receives 2^26 or more synapses; use a smaller tile";
+        if (L.tile_syn[t + 1] - L.tile_syn[t] >= ((int64_t)1 << 32)) {
+            err = "a tile receives 2^32 or more synapses; use a smaller tile";
             return -6;
         }
     }
@@ -203,31 +203,39 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
     std::vector<std::vector<bsim_range>> trng(NT);
     std::vector<std::vector<uint32_t>> toff(NT);
     const int32_t D = dmax;
+    const int32_t NW = BSIM_NWAVE;
+    const int32_t qw = tile / NW;   /* targets per quarter */
     int fail = 0;
 
 #pragma omp parallel for schedule(dynamic, 1)
     for (int32_t t = 0; t < NT; t++) {
         const int64_t lo = L.tile_syn[t], hi = L.tile_syn[t + 1], m = hi - lo;
-        /* key: pre (31 bits) | delay (6 bits) | rank within tile (26 bits) */
-        std::vector<uint64_t> key(m);
+        /* row key: pre (31 bits) | delay (6 bits) | quarter (2 bits); ties
+         * keep the creation rank */
+        struct Ent { uint64_t key; uint32_t rank; };
+        std::vector<Ent> ent(m);
         for (int64_t e = 0; e < m; e++) {
             const int64_t k = order[lo + e];
-            key[e] = ((uint64_t)(uint32_t)net.pre[k] << 32) | ((uint64_t)ds[k] << 26) | (uint64_t)e;
+            const uint32_t q = (uint32_t)((net.post[k] - t * tile) / qw);
+            ent[e].key = ((uint64_t)(uint32_t)net.pre[k] << 8) | ((uint64_t)ds[k] << 2) | q;
+            ent[e].rank = (uint32_t)e;
         }
-        std::sort(key.begin(), key.end());
+        std::sort(ent.begin(), ent.end(), [](const Ent &a, const Ent &b) {
+            return a.key < b.key || (a.key == b.key && a.rank < b.rank);
+        });
         for (int64_t e = 0; e < m; e++) {
-            const int64_t k = order[lo + (int64_t)(key[e] & ((1u << 26) - 1))];
+            const int64_t k = order[lo + ent[e].rank];
             L.syn_tgt[lo + e] = (uint16_t)(net.post[k] - t * tile);
             L.syn_w[lo + e] = w[k];
         }
         /* source ranges: runs of pres whose gaps are at most kRangeGap */
         int64_t e = 0;
         while (e < m) {
-            const int32_t p0 = (int32_t)(key[e] >> 32);
+            const int32_t p0 = (int32_t)(ent[e].key >> 8);
             int64_t f = e;
             int32_t plast = p0;
             while (f < m) {
-                const int32_t pf = (int32_t)(key[f] >> 32);
+                const int32_t pf = (int32_t)(ent[f].key >> 8);
                 if (pf - plast > kRangeGap) break;
                 plast = pf;
                 f++;
@@ -236,16 +244,18 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
             r.lo = p0;
             r.hi = plast + 1;
             r.off_base = (int64_t)toff[t].size();
-            const int64_t cnt = (int64_t)(r.hi - r.lo) * D + 1;
+            const int64_t cnt = (int64_t)(r.hi - r.lo) * D * NW + 1;
             const size_t base = toff[t].size();
             toff[t].resize(base + cnt);
             int64_t pos = e;
+            size_t x = base;
             for (int32_t p = r.lo; p < r.hi; p++) {
                 for (int32_t d = 1; d <= D; d++) {
-                    toff[t][base + (size_t)(p - r.lo) * D + (d - 1)] = (uint32_t)pos;
-                    while (pos < f && (int32_t)(key[pos] >> 32) == p &&
-                           (int32_t)((key[pos] >> 26) & 63) == d)
-                        pos++;
+                    for (int32_t q = 0; q < NW; q++) {
+                        const uint64_t key = ((uint64_t)(uint32_t)p << 8) | ((uint64_t)d << 2) | (uint64_t)q;
+                        toff[t][x++] = (uint32_t)pos;
+                        while (pos < f && ent[pos].key == key) pos++;
+                    }
                 }
             }
             toff[t][base + cnt - 1] = (uint32_t)pos;

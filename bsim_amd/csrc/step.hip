@@ -14,22 +14,24 @@
  *        and the ordered compaction of the tile's fired / active neurons into
  *        64-bit words by wavefront ballot.
  *
- * Phase (ii) pipeline for a tile:
- *   items     the tile's active pres, ascending: the "active" bitmap words of
- *             its source ranges are scanned by all 256 threads (popcount +
- *             block prefix) -- one round trip to memory;
- *   segments  for each item the history word gives its active delays; each
- *             (pre, delay) row of the tile is one segment [start, start+len)
- *             of the tile's synapse arrays, read from the row table -- two
- *             round trips;
- *   events    the concatenated segments are streamed in chunks of 512 events;
- *             the loads of chunk c+1 are in flight while chunk c is applied;
- *   apply     the events of a chunk are split, in stream order, into four
- *             queues by target quarter; wavefront q applies queue q to its own
- *             accumulators 64 events at a time.  Lanes whose accumulator also
- *             occurs in a lower lane wait: an LDS atomic-min on a per-
- *             accumulator tag elects the lowest lane, which adds; the others
- *             are carried, in order, to the front of the next batch.
+ * Phase (ii) for a tile, in windows of up to RCAP rows:
+ *   items   the tile's active pres, ascending: the "active" bitmap words of its
+ *           source ranges are scanned by all 256 threads (popcount + block
+ *           prefix);
+ *   rows    for each item its history word gives the active delays; each
+ *           active (pre, delay) row is staged in LDS as its four quarter
+ *           boundaries, read from the row table;
+ *   stream  wavefront q walks the staged rows in order and streams its own
+ *           quarter of each: the concatenated sub-rows form a stream of
+ *           positions, mapped to synapse addresses 64 at a time (row heads
+ *           found by ballot), 256 positions per step, the loads of step s+1
+ *           issued before step s is applied;
+ *   apply   the owner of local target x is lane x mod 64 of wavefront q.  For
+ *           each 64-event group the lanes ballot the six bits of the owner;
+ *           from the ballots every lane forms the mask of the group's events it
+ *           owns and applies them in ascending position -- stream order -- to
+ *           its accumulators.  No two lanes touch one accumulator, so there are
+ *           no atomics, no conflicts and no barriers between wavefronts.
  * Every accumulator thus receives its adds one at a time in stream order,
  * (pre, delay, creation) -- the creation order the host verified -- so the
  * sums are the reference's sequential fp32 sums, bit for bit.
@@ -44,18 +46,16 @@
 #include "layout.h"
 
 #define WAVE 64
-#define NWAVE 4
+#define NWAVE BSIM_NWAVE
 #define NTHREAD (WAVE * NWAVE)
-#define ITEMCAP 512      /* items staged per pass */
-#define SEGCAP 512       /* segments staged per pass */
-#define CHUNK 512        /* events per chunk: 2 per thread */
-#define SUBW (CHUNK / NWAVE)   /* events of a chunk loaded by one wavefront */
-#define BCAP 1024        /* boundary table: positions per part = BCAP * SUBW */
-#define TAG_TOP 0x03FFFFFFu
+#define ITEMCAP 512      /* items staged per pass: 2 per thread */
+#define RCAP 512         /* rows staged per window: 2 per thread */
+#define ROWW (NWAVE + 1) /* uint32 per staged row: quarter boundaries */
+#define NGRP 4           /* 64-position groups per streaming step */
+#define SSTEP (NGRP * WAVE)
 
-/* fixed LDS bytes besides the 16 bytes per target of acc + tag */
-#define LDS_FIXED (ITEMCAP * 4 + SEGCAP * 12 + BCAP * 2 + NWAVE * CHUNK * 8 + \
-                   16 * 4 * 4 + 16 * 4 * 4 + NWAVE * WAVE * 8 + 64)
+/* LDS bytes besides the 8 bytes per target of the accumulators */
+#define LDS_FIXED (4 * (RCAP * ROWW + ITEMCAP + NWAVE * SSTEP * 2 + 2 * NWAVE * WAVE + 16 + NWAVE))
 
 __device__ __forceinline__ uint32_t bern_hash(uint64_t seed, uint32_t id, uint32_t t)
 {
@@ -69,15 +69,17 @@ __device__ __forceinline__ uint32_t bern_hash(uint64_t seed, uint32_t id, uint32
 
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
-/* number of set bits of m below this lane */
-__device__ __forceinline__ uint32_t rank_below(uint64_t m)
-{
-    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
-}
-
 __device__ __forceinline__ uint32_t readlane(uint32_t x, int32_t l)
 {
     return __builtin_amdgcn_readlane(x, l);
+}
+
+/* orders this wavefront's LDS accesses (lanes exchange data through LDS) */
+__device__ __forceinline__ void wave_sync()
+{
+    __builtin_amdgcn_fence(__ATOMIC_RELEASE, "wavefront");
+    __builtin_amdgcn_wave_barrier();
+    __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
 /* inclusive prefix sum over the wavefront */
@@ -114,261 +116,200 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t *total,
     return before + v - x;
 }
 
-/* Per-wavefront accumulation state (registers). */
-struct Acc {
-    uint32_t round, ncarry, ckey;
-    float cw;
-};
-
-/* Applies up to 64 events (lanes below a.ncarry hold carried events, lanes
- * flagged `fresh` new ones) to the accumulators; events that must wait are
- * left, in order, in lanes [0, a.ncarry). */
-__device__ __forceinline__ void apply_batch(float *acc, uint32_t *tag, uint2 *carry_buf, bool fresh,
-                                            uint32_t nkey, float nw, Acc &a)
-{
-    const uint32_t lane = threadIdx.x & (WAVE - 1);
-    const bool valid = fresh || lane < a.ncarry;
-    const uint32_t key = fresh ? nkey : a.ckey;
-    const float w = fresh ? nw : a.cw;
-    const uint32_t tv = ((TAG_TOP - a.round) << 6) | lane;
-    a.round++;
-    if (valid) atomicMin(&tag[key], tv);
-    __builtin_amdgcn_wave_barrier();
-    bool leader = false;
-    if (valid) leader = (tag[key] == tv);
-    if (leader) {
-        const float s = acc[key];
-        acc[key] = s + w;
-    }
-    __builtin_amdgcn_wave_barrier();
-    const uint64_t lose = ballot64(valid && !leader);
-    if (lose) {
-        if (valid && !leader) carry_buf[rank_below(lose)] = make_uint2(key, __float_as_uint(w));
-        __builtin_amdgcn_wave_barrier();
-        a.ncarry = (uint32_t)__popcll(lose);
-        if (lane < a.ncarry) {
-            const uint2 c = carry_buf[lane];
-            a.ckey = c.x;
-            a.cw = __uint_as_float(c.y);
-        }
-        __builtin_amdgcn_wave_barrier();
-    } else {
-        a.ncarry = 0;
-    }
-}
-
 struct Shared {
-    float *acc;          /* [2T]: E at 2i, I at 2i+1 */
-    uint32_t *tag;       /* [2T] */
+    float *accE, *accI;  /* [T] accumulators by local target */
+    uint32_t *rows;      /* [RCAP][ROWW] quarter boundaries of the staged rows */
     uint32_t *items;     /* [ITEMCAP] pre ids */
-    uint32_t *sst, *sln, *spos;   /* [SEGCAP] segment start / length / stream position */
-    uint16_t *bseg;      /* [BCAP] segment holding position part0 + b * SUBW */
-    uint2 *queue;        /* [NWAVE][CHUNK] (key, weight bits) */
-    uint32_t *gcnt;      /* [16][4] events per (load group, owner) */
-    uint32_t *gbase;     /* [16][4] exclusive prefix over load groups */
-    uint2 *carry;        /* [NWAVE][WAVE] */
+    uint2 *stage;        /* [NWAVE][SSTEP] (local target in quarter, weight bits) */
+    uint32_t *hflag;     /* [NWAVE][WAVE] row-head flags of a 64-position group */
+    uint32_t *hbase;     /* [NWAVE][WAVE] row-head address bases */
     uint32_t *scr;       /* [16] scratch */
+    uint32_t *tot;       /* [NWAVE] stream length of each quarter in the window */
 };
 
-/* Event loader state of one chunk: two events per thread. */
-struct Chunk {
-    uint32_t tl[2];
-    float w[2];
-    bool v[2];
+/* A window of 64 staged rows in the lanes of a wavefront: lane j holds row
+ * rw + j of the wavefront's sub-row stream. */
+struct RowWin {
+    int32_t rw;      /* first row (uniform) */
+    uint32_t pos;    /* stream position of the sub-row */
+    uint32_t len;    /* sub-row length */
+    uint32_t base;   /* synapse index of the sub-row minus pos */
+    uint32_t end;    /* stream position after the window (uniform) */
 };
 
-/* Issues the loads of chunk c of the current part: wavefront w loads stream
- * positions [pos0 + w * SUBW, + SUBW), in two 64-lane groups. */
-__device__ __forceinline__ void load_chunk(const Shared &S, Chunk &ch, uint32_t pos0, uint32_t pend,
-                                           uint32_t part0, int32_t nseg, const uint16_t *tgt,
-                                           const float *wts)
+__device__ __forceinline__ void load_win(const Shared &S, RowWin &W, int32_t rw, uint32_t pos0,
+                                         int32_t nrows)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
-    const uint32_t w = threadIdx.x / WAVE;
-    const uint32_t base = pos0 + w * SUBW;
-    ch.v[0] = ch.v[1] = false;
-    ch.tl[0] = ch.tl[1] = 0;
-    ch.w[0] = ch.w[1] = 0.0f;
-    if (base >= pend) return;                       /* uniform per wavefront */
-    int32_t wb = (int32_t)S.bseg[(base - part0) / SUBW];   /* first segment of the window */
-    uint32_t st = 0, ps = 0, ln = 0;
-    if (wb + (int32_t)lane < nseg) {
-        st = S.sst[wb + lane];
-        ps = S.spos[wb + lane];
-        ln = S.sln[wb + lane];
+    const uint32_t q = threadIdx.x / WAVE;
+    const int32_t j = rw + (int32_t)lane;
+    uint32_t s0 = 0, s1 = 0;
+    if (j < nrows) {
+        s0 = S.rows[j * ROWW + q];
+        s1 = S.rows[j * ROWW + q + 1];
     }
-    int32_t jj = 0;
-    uint32_t addr[2] = {0, 0};
-#pragma unroll
-    for (int g = 0; g < 2; g++) {
-        const uint32_t qlo = base + g * WAVE;
-        const uint32_t qhi = min(qlo + WAVE, pend);
-        const uint32_t q = qlo + lane;
-        const bool v = qlo < pend && q < qhi;
-        ch.v[g] = v;
-        if (qlo >= pend) continue;                  /* uniform */
-        while (true) {
-            if (jj >= WAVE) {                       /* slide the register window */
-                wb += WAVE;
-                jj = 0;
-                st = ps = ln = 0;
-                if (wb + (int32_t)lane < nseg) {
-                    st = S.sst[wb + lane];
-                    ps = S.spos[wb + lane];
-                    ln = S.sln[wb + lane];
-                }
-            }
-            if (wb + jj >= nseg) break;
-            const uint32_t e = readlane(ps, jj);
-            if (e >= qhi) break;
-            const uint32_t l = readlane(ln, jj);
-            if (v && q >= e && q < e + l) addr[g] = readlane(st, jj) + (q - e);
-            if (e + l > qhi) break;                 /* continues into the next group */
-            jj++;
-        }
-    }
-#pragma unroll
-    for (int g = 0; g < 2; g++) {
-        if (ch.v[g]) {
-            ch.tl[g] = tgt[addr[g]];
-            ch.w[g] = wts[addr[g]];
-        }
-    }
+    W.rw = rw;
+    W.len = s1 - s0;
+    const uint32_t incl = wave_incl_scan(W.len);
+    W.pos = pos0 + incl - W.len;
+    W.base = s0 - W.pos;
+    W.end = pos0 + readlane(incl, WAVE - 1);
 }
 
-/* Splits a loaded chunk into the four owner queues in stream order and has
- * every wavefront apply its queue.  Owner of local target tl: tl / tw. */
-__device__ __forceinline__ void apply_chunk(const Shared &S, const Chunk &ch, uint32_t tw, Acc &a)
+/* Maps stream positions [b, b + 64) to synapse indices: lane i gets position
+ * b + i.  cur is the address base of the row covering position b - 1.
+ * Returns false for positions past the end of the stream. */
+__device__ __forceinline__ bool map_group(const Shared &S, RowWin &W, uint32_t b, uint32_t &cur,
+                                          uint32_t &addr, int32_t nrows)
 {
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = tid & (WAVE - 1);
-    const uint32_t w = tid / WAVE;
-    uint32_t own[2], key[2];
-    uint64_t b0[2], b1[2], b2[2], b3[2];   /* per owner ballots (named: no indexed array) */
-#pragma unroll
-    for (int g = 0; g < 2; g++) {
-        own[g] = ch.v[g] ? ch.tl[g] / tw : NWAVE;
-        key[g] = 2u * ch.tl[g] + ((ch.w[g] >= 0.0f) ? 0u : 1u);
-        b0[g] = ballot64(own[g] == 0u);
-        b1[g] = ballot64(own[g] == 1u);
-        b2[g] = ballot64(own[g] == 2u);
-        b3[g] = ballot64(own[g] == 3u);
-        if (lane < NWAVE) {
-            /* lane o writes the count of owner o for load group (w, g) */
-            const uint64_t b = lane == 0 ? b0[g] : lane == 1 ? b1[g] : lane == 2 ? b2[g] : b3[g];
-            S.gcnt[(w * 2 + g) * NWAVE + lane] = (uint32_t)__popcll(b);
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t q = threadIdx.x / WAVE;
+    uint32_t *hf = S.hflag + q * WAVE;
+    uint32_t *hb = S.hbase + q * WAVE;
+    const uint32_t p = b + lane;
+    const uint64_t le = (lane == WAVE - 1) ? ~0ull : ((2ull << lane) - 1ull);
+    bool res = false;
+    uint32_t mb = 0;
+    while (true) {
+        /* heads: sub-rows of the window that start inside the group */
+        hf[lane] = 0u;
+        wave_sync();
+        const uint32_t rel = W.pos - b;
+        if (W.len > 0 && rel < WAVE) {
+            hf[rel] = 1u;
+            hb[rel] = W.base;
         }
-    }
-    __syncthreads();
-    if (tid < 8 * NWAVE) {
-        /* thread (G, o): prefix of owner o over load groups before G */
-        const uint32_t G = tid / NWAVE, o = tid % NWAVE;
-        uint32_t s = 0;
-        for (uint32_t x = 0; x < G; x++) s += S.gcnt[x * NWAVE + o];
-        S.gbase[G * NWAVE + o] = s;
-        if (G == 7) S.scr[8 + o] = s + S.gcnt[7 * NWAVE + o];   /* queue length */
-    }
-    __syncthreads();
-#pragma unroll
-    for (int g = 0; g < 2; g++) {
-        if (ch.v[g]) {
-            const uint32_t o = own[g];
-            const uint64_t b = o == 0 ? b0[g] : o == 1 ? b1[g] : o == 2 ? b2[g] : b3[g];
-            const uint32_t pos = S.gbase[(w * 2 + g) * NWAVE + o] + rank_below(b);
-            S.queue[o * CHUNK + pos] = make_uint2(key[g], __float_as_uint(ch.w[g]));
-        }
-    }
-    __syncthreads();
-    /* wavefront w applies queue w */
-    const uint32_t qlen = S.scr[8 + w];
-    const uint2 *qu = S.queue + w * CHUNK;
-    uint2 *carry = S.carry + w * WAVE;
-    uint32_t qp = 0;
-    while (qp < qlen) {
-        const uint32_t take = min(WAVE - a.ncarry, qlen - qp);
-        const bool fresh = lane >= a.ncarry && lane < a.ncarry + take;
-        uint32_t nk = 0;
-        float nwt = 0.0f;
-        if (fresh) {
-            const uint2 e = qu[qp + lane - a.ncarry];
-            nk = e.x;
-            nwt = __uint_as_float(e.y);
-        }
-        qp += take;
-        apply_batch(S.acc, S.tag, carry, fresh, nk, nwt, a);
-    }
-}
-
-/* Streams the events of the staged segments [0, nseg) and applies them. */
-__device__ void flush_segments(const Shared &S, int32_t nseg, uint32_t tw, const uint16_t *tgt,
-                               const float *wts, Acc &a, int64_t &events)
-{
-    const uint32_t tid = threadIdx.x;
-    /* stream positions: exclusive prefix of the segment lengths */
-    uint32_t len[2], lp[2], cnt = 0;
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-        const int32_t s = 2 * tid + j;
-        len[j] = s < nseg ? S.sln[s] : 0u;
-        lp[j] = cnt;
-        cnt += len[j];
-    }
-    uint32_t E;
-    const uint32_t ex = block_excl_scan(cnt, &E, S.scr);
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-        const int32_t s = 2 * tid + j;
-        if (s < nseg) S.spos[s] = ex + lp[j];
-    }
-    events += E;
-    for (uint32_t part0 = 0; part0 < E; part0 += BCAP * SUBW) {
-        const uint32_t pend = min(E, part0 + BCAP * SUBW);
-        /* boundary table: segment holding each position part0 + b * SUBW */
-#pragma unroll
-        for (int j = 0; j < 2; j++) {
-            const int32_t s = 2 * tid + j;
-            if (s < nseg && len[j] > 0) {
-                const uint32_t p0 = ex + lp[j], p1 = p0 + len[j];
-                const uint32_t lo = max(p0, part0), hi = min(p1, pend);
-                if (lo < hi) {
-                    uint32_t b = (lo - part0 + SUBW - 1) / SUBW;
-                    for (; part0 + b * SUBW < hi; b++) S.bseg[b] = (uint16_t)s;
-                }
+        wave_sync();
+        const uint64_t H = ballot64(hf[lane] != 0u);
+        if (!res && p < W.end) {
+            const uint64_t hq = H & le;
+            if (hq) {
+                mb = hb[63 - __builtin_clzll(hq)];
+            } else {
+                mb = cur;   /* the row covering b - 1 continues */
             }
+            res = true;
         }
-        __syncthreads();
-        /* two chunk buffers: the loads of the next chunk are issued before
-         * the current one is applied */
-        Chunk A, B;
-        load_chunk(S, A, part0, pend, part0, nseg, tgt, wts);
-        uint32_t c0 = part0;
-        while (true) {
-            if (c0 + CHUNK < pend) load_chunk(S, B, c0 + CHUNK, pend, part0, nseg, tgt, wts);
-            apply_chunk(S, A, tw, a);          /* barriers inside */
-            c0 += CHUNK;
-            if (c0 >= pend) break;
-            if (c0 + CHUNK < pend) load_chunk(S, A, c0 + CHUNK, pend, part0, nseg, tgt, wts);
-            apply_chunk(S, B, tw, a);
-            c0 += CHUNK;
-            if (c0 >= pend) break;
+        wave_sync();
+        if (W.end >= b + WAVE) break;             /* uniform */
+        if (W.rw + WAVE >= nrows) break;          /* rows exhausted: stream ends */
+        load_win(S, W, W.rw + WAVE, W.end, nrows);
+    }
+    cur = readlane(mb, WAVE - 1);
+    addr = mb + p;
+    return res;
+}
+
+/* Loaded events of one streaming step: NGRP groups of 64. */
+struct Step {
+    uint32_t tl[NGRP];
+    float w[NGRP];
+    bool v[NGRP];
+};
+
+__device__ __forceinline__ void load_step(const Shared &S, RowWin &W, uint32_t b0, uint32_t tot,
+                                          uint32_t &cur, int32_t nrows, const uint16_t *tgt,
+                                          const float *wts, Step &st)
+{
+#pragma unroll
+    for (int g = 0; g < NGRP; g++) {
+        const uint32_t b = b0 + g * WAVE;
+        uint32_t addr = 0;
+        bool v = false;
+        if (b < tot) v = map_group(S, W, b, cur, addr, nrows);   /* uniform branch */
+        st.v[g] = v;
+        st.tl[g] = 0;
+        st.w[g] = 0.0f;
+        if (v) {
+            st.tl[g] = tgt[addr];
+            st.w[g] = wts[addr];
         }
-        __syncthreads();
     }
 }
 
-/* Turns the staged items [0, nitems) of source range r into segments and
- * streams them.  Items are pre ids, ascending. */
+/* Applies one streaming step: every lane applies, in stream order, the events
+ * whose target it owns. */
+__device__ __forceinline__ void apply_step(const Shared &S, const Step &st, uint32_t qbase)
+{
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t q = threadIdx.x / WAVE;
+    uint2 *stg = S.stage + q * SSTEP;
+    float *aE = S.accE + qbase;
+    float *aI = S.accI + qbase;
+    uint64_t M[NGRP];
+#pragma unroll
+    for (int g = 0; g < NGRP; g++) {
+        const bool v = st.v[g];
+        const uint32_t li = st.tl[g] - qbase;     /* local target within the quarter */
+        stg[g * WAVE + lane] = make_uint2(li, __float_as_uint(st.w[g]));
+        uint64_t m = ballot64(v);
+#pragma unroll
+        for (int bit = 0; bit < 6; bit++) {
+            const uint64_t bb = ballot64(v && ((li >> bit) & 1u));
+            m &= ((lane >> bit) & 1u) ? bb : ~bb;
+        }
+        M[g] = m;
+    }
+    wave_sync();
+    uint64_t m0 = M[0], m1 = M[1], m2 = M[2], m3 = M[3];
+    while ((m0 | m1 | m2 | m3) != 0ull) {
+        uint32_t g;
+        uint64_t mm;
+        if (m0) {
+            g = 0; mm = m0; m0 &= m0 - 1;
+        } else if (m1) {
+            g = 1; mm = m1; m1 &= m1 - 1;
+        } else if (m2) {
+            g = 2; mm = m2; m2 &= m2 - 1;
+        } else {
+            g = 3; mm = m3; m3 &= m3 - 1;
+        }
+        const uint2 e = stg[g * WAVE + (uint32_t)__builtin_ctzll(mm)];
+        const float x = __uint_as_float(e.y);
+        float *a = (x >= 0.0f) ? aE : aI;
+        a[e.x] = a[e.x] + x;
+    }
+    wave_sync();
+}
+
+/* Streams this wavefront's quarter of the staged rows [0, nrows). */
+__device__ void stream_rows(const Shared &S, int32_t nrows, uint32_t tot, uint32_t qbase,
+                            const uint16_t *tgt, const float *wts)
+{
+    if (tot == 0) return;                                     /* uniform */
+    RowWin W;
+    load_win(S, W, 0, 0, nrows);
+    uint32_t cur = 0;
+    Step A, B;
+    load_step(S, W, 0, tot, cur, nrows, tgt, wts, A);
+    uint32_t b = 0;
+    while (true) {
+        if (b + SSTEP < tot) load_step(S, W, b + SSTEP, tot, cur, nrows, tgt, wts, B);
+        apply_step(S, A, qbase);
+        b += SSTEP;
+        if (b >= tot) break;
+        if (b + SSTEP < tot) load_step(S, W, b + SSTEP, tot, cur, nrows, tgt, wts, A);
+        apply_step(S, B, qbase);
+        b += SSTEP;
+        if (b >= tot) break;
+    }
+}
+
+/* Turns the staged items [0, nitems) of source range r into rows and streams
+ * them.  Items are pre ids, ascending. */
 __device__ void flush_items(const Shared &S, const bsim_dev &d, int32_t nitems, const bsim_range &r,
-                            int32_t par, uint32_t tw, const uint16_t *tgt, const float *wts, Acc &a,
+                            int32_t par, uint32_t qbase, const uint16_t *tgt, const float *wts,
                             int64_t &events)
 {
     const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & (WAVE - 1);
+    const uint32_t q = tid / WAVE;
     const int32_t D = d.dmax;
     const uint64_t dmask = (D >= 63) ? (~0ull >> 1) : ((1ull << D) - 1ull);
     const uint64_t *hist = d.hist + (int64_t)par * d.n;
     uint32_t p[2];
     uint64_t m[2];
-    uint32_t nb[2], lp[2], cnt = 0;
+    uint32_t lp[2], cnt = 0;
 #pragma unroll
     for (int j = 0; j < 2; j++) {
         const int32_t i = 2 * tid + j;
@@ -377,14 +318,14 @@ __device__ void flush_items(const Shared &S, const bsim_dev &d, int32_t nitems, 
     }
 #pragma unroll
     for (int j = 0; j < 2; j++) {
-        nb[j] = (uint32_t)__popcll(m[j]);
         lp[j] = cnt;
-        cnt += nb[j];
+        cnt += (uint32_t)__popcll(m[j]);
     }
     uint32_t tot;
     const uint32_t ex = block_excl_scan(cnt, &tot, S.scr);
-    for (uint32_t emitted = 0; emitted < tot; emitted += SEGCAP) {
-        const uint32_t wend = min(tot, emitted + SEGCAP);
+    for (uint32_t emitted = 0; emitted < tot; emitted += RCAP) {
+        const uint32_t wend = min(tot, emitted + RCAP);
+        uint32_t qlen[NWAVE] = {0, 0, 0, 0};
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             uint64_t mm = m[j];
@@ -393,16 +334,30 @@ __device__ void flush_items(const Shared &S, const bsim_dev &d, int32_t nitems, 
                 const int32_t bit = __builtin_ctzll(mm);
                 mm &= mm - 1;
                 if (rank >= emitted && rank < wend) {
-                    const int64_t row = r.off_base + (int64_t)(p[j] - (uint32_t)r.lo) * D + bit;
-                    const uint32_t o0 = d.off_tab[row], o1 = d.off_tab[row + 1];
-                    S.sst[rank - emitted] = o0;
-                    S.sln[rank - emitted] = o1 - o0;
+                    const int64_t e0 = r.off_base +
+                                       ((int64_t)(p[j] - (uint32_t)r.lo) * D + bit) * NWAVE;
+                    uint32_t *row = S.rows + (rank - emitted) * ROWW;
+                    uint32_t o[ROWW];
+#pragma unroll
+                    for (int x = 0; x < ROWW; x++) o[x] = d.off_tab[e0 + x];
+#pragma unroll
+                    for (int x = 0; x < ROWW; x++) row[x] = o[x];
+#pragma unroll
+                    for (int x = 0; x < NWAVE; x++) qlen[x] += o[x + 1] - o[x];
                 }
                 rank++;
             }
         }
+#pragma unroll
+        for (int x = 0; x < NWAVE; x++)
+            if (qlen[x]) atomicAdd(&S.tot[x], qlen[x]);
         __syncthreads();
-        flush_segments(S, (int32_t)(wend - emitted), tw, tgt, wts, a, events);
+        const uint32_t qtot = S.tot[q];
+        wave_sync();
+        if (lane == 0) S.tot[q] = 0u;       /* read by this wavefront only */
+        events += qtot;
+        stream_rows(S, (int32_t)(wend - emitted), qtot, qbase, tgt, wts);
+        __syncthreads();
     }
 }
 
@@ -414,27 +369,25 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & (WAVE - 1);
     const uint32_t w = tid / WAVE;
-    const uint32_t tw = (uint32_t)T / NWAVE;
+    const uint32_t qbase = w * ((uint32_t)T / NWAVE);
 
     Shared S;
-    S.acc = reinterpret_cast<float *>(lds);
-    S.tag = lds + 2 * T;
-    uint32_t *p = S.tag + 2 * T;
-    S.queue = reinterpret_cast<uint2 *>(p);           p += NWAVE * CHUNK * 2;
-    S.carry = reinterpret_cast<uint2 *>(p);           p += NWAVE * WAVE * 2;
+    uint32_t *p = lds;
+    S.accE = reinterpret_cast<float *>(p);            p += T;
+    S.accI = reinterpret_cast<float *>(p);            p += T;
+    S.stage = reinterpret_cast<uint2 *>(p);           p += NWAVE * SSTEP * 2;
+    S.rows = p;                                       p += RCAP * ROWW;
     S.items = p;                                      p += ITEMCAP;
-    S.sst = p;                                        p += SEGCAP;
-    S.sln = p;                                        p += SEGCAP;
-    S.spos = p;                                       p += SEGCAP;
-    S.gcnt = p;                                       p += 16 * NWAVE;
-    S.gbase = p;                                      p += 16 * NWAVE;
+    S.hflag = p;                                      p += NWAVE * WAVE;
+    S.hbase = p;                                      p += NWAVE * WAVE;
     S.scr = p;                                        p += 16;
-    S.bseg = reinterpret_cast<uint16_t *>(p);
+    S.tot = p;
 
-    for (int32_t i = tid; i < 2 * T; i += NTHREAD) {
-        S.acc[i] = 0.0f;
-        S.tag[i] = 0xFFFFFFFFu;
+    for (int32_t i = tid; i < T; i += NTHREAD) {
+        S.accE[i] = 0.0f;
+        S.accI[i] = 0.0f;
     }
+    if (tid < NWAVE) S.tot[tid] = 0u;
     __syncthreads();
 
     /* ---- phase (ii) of cycle t-1 ---------------------------------------- */
@@ -444,11 +397,6 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
         const uint16_t *tgt = d.syn_tgt + syn0;
         const float *wts = d.syn_w + syn0;
         const uint64_t *act = d.active + (int64_t)par * d.nwords;
-        Acc a;
-        a.round = 0;
-        a.ncarry = 0;
-        a.ckey = 0;
-        a.cw = 0.0f;
         int64_t events = 0;
         for (int32_t ri = d.tile_rng[k]; ri < d.tile_rng[k + 1]; ri++) {
             const bsim_range r = d.rng[ri];
@@ -479,18 +427,20 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
                     nitems += (int32_t)n;
                     emitted += n;
                     if (nitems == ITEMCAP) {
-                        flush_items(S, d, nitems, r, par, tw, tgt, wts, a, events);
+                        flush_items(S, d, nitems, r, par, qbase, tgt, wts, events);
                         nitems = 0;
                     }
                 }
             }
-            if (nitems > 0) flush_items(S, d, nitems, r, par, tw, tgt, wts, a, events);
+            if (nitems > 0) flush_items(S, d, nitems, r, par, qbase, tgt, wts, events);
         }
-        /* drain the carried events */
-        while (__builtin_amdgcn_readfirstlane(a.ncarry) > 0)
-            apply_batch(S.acc, S.tag, S.carry + w * WAVE, false, 0u, 0.0f, a);
-        if (tid == 0) d.ev_count[k] += events;
+        if (lane == 0) S.scr[8 + w] = (uint32_t)events;
         __syncthreads();
+        if (tid == 0) {
+            int64_t e = 0;
+            for (int i = 0; i < NWAVE; i++) e += S.scr[8 + i];
+            d.ev_count[k] += e;
+        }
     }
 
     /* ---- phase (i) of cycle t -------------------------------------------- */
@@ -510,8 +460,8 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
             if (valid) {
                 const int8_t kind = d.kind[nid];
                 if (kind == BSIM_KIND_LIF) {
-                    const float inE = S.acc[2 * (i0 + lane)];
-                    const float inI = S.acc[2 * (i0 + lane) + 1];
+                    const float inE = S.accE[i0 + lane];
+                    const float inI = S.accI[i0 + lane];
                     const int32_t rs = d.refrac[nid];
                     if (rs > 0) {
                         d.refrac[nid] = rs - 1;
@@ -561,7 +511,7 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
 
 extern "C" size_t bsim_step_lds_bytes(int32_t tile)
 {
-    return (size_t)tile * 16 + LDS_FIXED;
+    return (size_t)tile * 8 + LDS_FIXED;
 }
 
 extern "C" hipError_t bsim_launch_step(const bsim_dev *d, int32_t t, int32_t flags,

@@ -43,7 +43,7 @@ def test_constants_match_reference(name):
     np.testing.assert_array_equal(rt[lif], g["refrac_time"][lif])
 
 
-def test_layout_is_pre_delay_sorted():
+def test_layout_is_pre_delay_quarter_sorted():
     spec = netspec.random_net(5, n_lif=700, p_conn=0.03, dmax=7, allow_dup=True)
     net = api.Network.from_spec(spec)
     tile = 256
@@ -51,7 +51,8 @@ def test_layout_is_pre_delay_sorted():
     w_ref, ds = oracle.synapse_reset(spec, DT)
     for k in range(L["ntiles"]):
         sel = np.nonzero(spec.post // tile == k)[0]          # creation order
-        order = sel[np.lexsort((sel, ds[sel], spec.pre[sel]))]
+        quarter = (spec.post[sel] - k * tile) // (tile // 4)
+        order = sel[np.lexsort((sel, quarter, ds[sel], spec.pre[sel]))]
         a, b = L["tile_syn"][k], L["tile_syn"][k + 1]
         np.testing.assert_array_equal(L["syn_tgt"][a:b], spec.post[order] - k * tile)
         assert np.array_equal(L["syn_w"][a:b].view(np.uint32), w_ref[order].view(np.uint32))
