@@ -13,6 +13,7 @@
  *        which consumes exactly those accumulators, the spike history update
  *        and the ordered compaction of the tile's fired / active neurons into
  *        64-bit words by wavefront ballot.
+ * The kernel is instantiated for S = TILE / 256 accumulator slots per lane.
  *
  * Phase (ii) for a tile, in windows of up to RCAP rows:
  *   items   the tile's active pres, ascending: the "active" bitmap words of its
@@ -26,12 +27,13 @@
  *           positions, mapped to synapse addresses 64 at a time (row heads
  *           found by ballot), 256 positions per step, the loads of step s+1
  *           issued before step s is applied;
- *   apply   the owner of local target x is lane x mod 64 of wavefront q.  For
- *           each 64-event group the lanes ballot the six bits of the owner;
- *           from the ballots every lane forms the mask of the group's events it
- *           owns and applies them in ascending position -- stream order -- to
- *           its accumulators.  No two lanes touch one accumulator, so there are
- *           no atomics, no conflicts and no barriers between wavefronts.
+ *   apply   the owner of local target x is lane x mod 64 of wavefront q, which
+ *           keeps x's E and I accumulators in registers.  For each 64-event
+ *           group the lanes ballot the six bits of the owner; from the ballots
+ *           every lane forms the mask of the group's events it owns and applies
+ *           them in ascending position -- stream order.  No two lanes touch one
+ *           accumulator, so there are no atomics, no conflicts and no barriers
+ *           between wavefronts; each wavefront then updates its own quarter.
  * Every accumulator thus receives its adds one at a time in stream order,
  * (pre, delay, creation) -- the creation order the host verified -- so the
  * sums are the reference's sequential fp32 sums, bit for bit.
@@ -54,7 +56,7 @@
 #define NGRP 4           /* 64-position groups per streaming step */
 #define SSTEP (NGRP * WAVE)
 
-/* LDS bytes besides the 8 bytes per target of the accumulators */
+/* LDS bytes of a workgroup */
 #define LDS_FIXED (4 * (RCAP * ROWW + ITEMCAP + NWAVE * SSTEP * 2 + 2 * NWAVE * WAVE + 16 + NWAVE))
 
 __device__ __forceinline__ uint32_t bern_hash(uint64_t seed, uint32_t id, uint32_t t)
@@ -82,17 +84,25 @@ __device__ __forceinline__ void wave_sync()
     __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "wavefront");
 }
 
-/* inclusive prefix sum over the wavefront */
+/* DPP controls (GFX9 encoding): shift right within a row of 16 lanes, and
+ * broadcast lane 15 / 31 into the following rows */
+#define DPP_ROW_SHR(n) (0x110 + (n))
+#define DPP_ROW_BCAST15 0x142
+#define DPP_ROW_BCAST31 0x143
+
+/* inclusive prefix sum over the wavefront: four in-row shifts, then the row
+ * totals are carried across rows by two broadcasts; lanes without a source
+ * read 0 */
 __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x)
 {
-    const int lane = threadIdx.x & (WAVE - 1);
-    uint32_t v = x;
-#pragma unroll
-    for (int off = 1; off < WAVE; off <<= 1) {
-        const uint32_t y = __shfl_up(v, off, WAVE);
-        if (lane >= off) v += y;
-    }
-    return v;
+    int v = (int)x;
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(1), 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(2), 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(4), 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_SHR(8), 0xf, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_BCAST15, 0xa, 0xf, false);
+    v += __builtin_amdgcn_update_dpp(0, v, DPP_ROW_BCAST31, 0xc, 0xf, false);
+    return (uint32_t)v;
 }
 
 /* exclusive prefix sum over the workgroup; *total = sum over all threads.
@@ -117,7 +127,6 @@ __device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t *total,
 }
 
 struct Shared {
-    float *accE, *accI;  /* [T] accumulators by local target */
     uint32_t *rows;      /* [RCAP][ROWW] quarter boundaries of the staged rows */
     uint32_t *items;     /* [ITEMCAP] pre ids */
     uint2 *stage;        /* [NWAVE][SSTEP] (local target in quarter, weight bits) */
@@ -207,9 +216,12 @@ struct Step {
     bool v[NGRP];
 };
 
+/* The loads are issued unconditionally (lanes without an event read the
+ * tile's first synapse), so that the compiler can count them exactly and the
+ * apply of the previous step waits only for its own loads. */
 __device__ __forceinline__ void load_step(const Shared &S, RowWin &W, uint32_t b0, uint32_t tot,
-                                          uint32_t &cur, int32_t nrows, const uint16_t *tgt,
-                                          const float *wts, Step &st)
+                                          uint32_t &cur, int32_t nrows, uint32_t nsyn,
+                                          const uint16_t *tgt, const float *wts, Step &st)
 {
 #pragma unroll
     for (int g = 0; g < NGRP; g++) {
@@ -217,25 +229,28 @@ __device__ __forceinline__ void load_step(const Shared &S, RowWin &W, uint32_t b
         uint32_t addr = 0;
         bool v = false;
         if (b < tot) v = map_group(S, W, b, cur, addr, nrows);   /* uniform branch */
+        addr = (v && addr < nsyn) ? addr : 0u;
         st.v[g] = v;
-        st.tl[g] = 0;
-        st.w[g] = 0.0f;
-        if (v) {
-            st.tl[g] = tgt[addr];
-            st.w[g] = wts[addr];
-        }
+        st.tl[g] = tgt[addr];
+        st.w[g] = wts[addr];
     }
 }
 
+/* E and I accumulators of the S local targets lane, lane + 64, ... of the
+ * wavefront's quarter */
+template <int S_>
+struct Acc {
+    float e[S_], i[S_];
+};
+
 /* Applies one streaming step: every lane applies, in stream order, the events
  * whose target it owns. */
-__device__ __forceinline__ void apply_step(const Shared &S, const Step &st, uint32_t qbase)
+template <int S_>
+__device__ __forceinline__ void apply_step(const Shared &S, const Step &st, uint32_t qbase, Acc<S_> &acc)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     const uint32_t q = threadIdx.x / WAVE;
     uint2 *stg = S.stage + q * SSTEP;
-    float *aE = S.accE + qbase;
-    float *aI = S.accI + qbase;
     uint64_t M[NGRP];
 #pragma unroll
     for (int g = 0; g < NGRP; g++) {
@@ -266,30 +281,41 @@ __device__ __forceinline__ void apply_step(const Shared &S, const Step &st, uint
         }
         const uint2 e = stg[g * WAVE + (uint32_t)__builtin_ctzll(mm)];
         const float x = __uint_as_float(e.y);
-        float *a = (x >= 0.0f) ? aE : aI;
-        a[e.x] = a[e.x] + x;
+        const uint32_t slot = e.x >> 6;
+        const bool exc = x >= 0.0f;
+#pragma unroll
+        for (int j = 0; j < S_; j++) {
+            if (slot == (uint32_t)j) {
+                if (exc)
+                    acc.e[j] = acc.e[j] + x;
+                else
+                    acc.i[j] = acc.i[j] + x;
+            }
+        }
     }
     wave_sync();
 }
 
-/* Streams this wavefront's quarter of the staged rows [0, nrows). */
+/* Streams this wavefront's quarter of the staged rows [0, nrows).  The loads
+ * of step s+1 are in flight while step s is applied. */
+template <int S_>
 __device__ void stream_rows(const Shared &S, int32_t nrows, uint32_t tot, uint32_t qbase,
-                            const uint16_t *tgt, const float *wts)
+                            uint32_t nsyn, const uint16_t *tgt, const float *wts, Acc<S_> &acc)
 {
     if (tot == 0) return;                                     /* uniform */
     RowWin W;
     load_win(S, W, 0, 0, nrows);
     uint32_t cur = 0;
     Step A, B;
-    load_step(S, W, 0, tot, cur, nrows, tgt, wts, A);
+    load_step(S, W, 0, tot, cur, nrows, nsyn, tgt, wts, A);
     uint32_t b = 0;
     while (true) {
-        if (b + SSTEP < tot) load_step(S, W, b + SSTEP, tot, cur, nrows, tgt, wts, B);
-        apply_step(S, A, qbase);
+        load_step(S, W, b + SSTEP, tot, cur, nrows, nsyn, tgt, wts, B);
+        apply_step<S_>(S, A, qbase, acc);
         b += SSTEP;
         if (b >= tot) break;
-        if (b + SSTEP < tot) load_step(S, W, b + SSTEP, tot, cur, nrows, tgt, wts, A);
-        apply_step(S, B, qbase);
+        load_step(S, W, b + SSTEP, tot, cur, nrows, nsyn, tgt, wts, A);
+        apply_step<S_>(S, B, qbase, acc);
         b += SSTEP;
         if (b >= tot) break;
     }
@@ -297,9 +323,10 @@ __device__ void stream_rows(const Shared &S, int32_t nrows, uint32_t tot, uint32
 
 /* Turns the staged items [0, nitems) of source range r into rows and streams
  * them.  Items are pre ids, ascending. */
+template <int S_>
 __device__ void flush_items(const Shared &S, const bsim_dev &d, int32_t nitems, const bsim_range &r,
-                            int32_t par, uint32_t qbase, const uint16_t *tgt, const float *wts,
-                            int64_t &events)
+                            int32_t par, uint32_t qbase, uint32_t nsyn, const uint16_t *tgt,
+                            const float *wts, int64_t &events, Acc<S_> &acc)
 {
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & (WAVE - 1);
@@ -356,11 +383,12 @@ __device__ void flush_items(const Shared &S, const bsim_dev &d, int32_t nitems, 
         wave_sync();
         if (lane == 0) S.tot[q] = 0u;       /* read by this wavefront only */
         events += qtot;
-        stream_rows(S, (int32_t)(wend - emitted), qtot, qbase, tgt, wts);
+        stream_rows<S_>(S, (int32_t)(wend - emitted), qtot, qbase, nsyn, tgt, wts, acc);
         __syncthreads();
     }
 }
 
+template <int S_>
 __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t t, int32_t flags)
 {
     extern __shared__ __align__(16) uint32_t lds[];
@@ -373,8 +401,6 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
 
     Shared S;
     uint32_t *p = lds;
-    S.accE = reinterpret_cast<float *>(p);            p += T;
-    S.accI = reinterpret_cast<float *>(p);            p += T;
     S.stage = reinterpret_cast<uint2 *>(p);           p += NWAVE * SSTEP * 2;
     S.rows = p;                                       p += RCAP * ROWW;
     S.items = p;                                      p += ITEMCAP;
@@ -383,17 +409,20 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
     S.scr = p;                                        p += 16;
     S.tot = p;
 
-    for (int32_t i = tid; i < T; i += NTHREAD) {
-        S.accE[i] = 0.0f;
-        S.accI[i] = 0.0f;
+    Acc<S_> acc;
+#pragma unroll
+    for (int j = 0; j < S_; j++) {
+        acc.e[j] = 0.0f;
+        acc.i[j] = 0.0f;
     }
-    if (tid < NWAVE) S.tot[tid] = 0u;
-    __syncthreads();
 
     /* ---- phase (ii) of cycle t-1 ---------------------------------------- */
     if (flags & 1) {
+        if (tid < NWAVE) S.tot[tid] = 0u;
+        __syncthreads();
         const int32_t par = (t - 1) & 1;
         const int64_t syn0 = d.tile_syn[k];
+        const uint32_t nsyn = (uint32_t)(d.tile_syn[k + 1] - syn0);
         const uint16_t *tgt = d.syn_tgt + syn0;
         const float *wts = d.syn_w + syn0;
         const uint64_t *act = d.active + (int64_t)par * d.nwords;
@@ -427,41 +456,37 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
                     nitems += (int32_t)n;
                     emitted += n;
                     if (nitems == ITEMCAP) {
-                        flush_items(S, d, nitems, r, par, qbase, tgt, wts, events);
+                        flush_items<S_>(S, d, nitems, r, par, qbase, nsyn, tgt, wts, events, acc);
                         nitems = 0;
                     }
                 }
             }
-            if (nitems > 0) flush_items(S, d, nitems, r, par, qbase, tgt, wts, events);
+            if (nitems > 0) flush_items<S_>(S, d, nitems, r, par, qbase, nsyn, tgt, wts, events, acc);
         }
-        if (lane == 0) S.scr[8 + w] = (uint32_t)events;
-        __syncthreads();
-        if (tid == 0) {
-            int64_t e = 0;
-            for (int i = 0; i < NWAVE; i++) e += S.scr[8 + i];
-            d.ev_count[k] += e;
-        }
+        if (lane == 0) atomicAdd((unsigned long long *)&d.ev_count[k], (unsigned long long)events);
     }
 
-    /* ---- phase (i) of cycle t -------------------------------------------- */
+    /* ---- phase (i) of cycle t: wavefront w updates its own quarter ---------- */
     if (flags & 2) {
         const int32_t D = d.dmax;
         const uint64_t dmask = (D >= 63) ? (~0ull >> 1) : ((1ull << D) - 1ull);
-        const int32_t base = k * T;
+        const int32_t base = k * T + (int32_t)qbase;
         const uint64_t *hist_in = d.hist + (int64_t)((t - 1) & 1) * d.n;
         uint64_t *hist_out = d.hist + (int64_t)(t & 1) * d.n;
         uint64_t *act_out = d.active + (int64_t)(t & 1) * d.nwords;
         const bool rec = d.raster != nullptr && t >= d.rec_t0 && t < d.rec_t0 + d.rec_steps;
-        for (int32_t i0 = (int32_t)w * WAVE; i0 < T; i0 += NTHREAD) {
-            const int32_t nid = base + i0 + (int32_t)lane;
+#pragma unroll
+        for (int j = 0; j < S_; j++) {
+            const int32_t n0 = base + j * WAVE;
+            const int32_t nid = n0 + (int32_t)lane;
             const bool valid = nid < d.n;
             bool fired = false;
             uint64_t h = 0;
             if (valid) {
                 const int8_t kind = d.kind[nid];
                 if (kind == BSIM_KIND_LIF) {
-                    const float inE = S.accE[i0 + lane];
-                    const float inI = S.accI[i0 + lane];
+                    const float inE = acc.e[j];
+                    const float inI = acc.i[j];
                     const int32_t rs = d.refrac[nid];
                     if (rs > 0) {
                         d.refrac[nid] = rs - 1;
@@ -501,9 +526,9 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
             }
             const uint64_t fm = ballot64(fired);
             const uint64_t am = ballot64(valid && ((h >> 1) & dmask) != 0);
-            if (lane == 0 && base + i0 < d.n) {
-                act_out[(base + i0) / 64] = am;
-                if (rec) d.raster[(int64_t)(t - d.rec_t0) * d.nwords + (base + i0) / 64] = fm;
+            if (lane == 0 && n0 < d.n) {
+                act_out[n0 / 64] = am;
+                if (rec) d.raster[(int64_t)(t - d.rec_t0) * d.nwords + n0 / 64] = fm;
             }
         }
     }
@@ -511,14 +536,26 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
 
 extern "C" size_t bsim_step_lds_bytes(int32_t tile)
 {
-    return (size_t)tile * 8 + LDS_FIXED;
+    (void)tile;
+    return LDS_FIXED;
 }
 
 extern "C" hipError_t bsim_launch_step(const bsim_dev *d, int32_t t, int32_t flags,
                                        hipStream_t stream)
 {
     if (d->ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(bsim_step_kernel, dim3(d->ntiles), dim3(NTHREAD),
-                       bsim_step_lds_bytes(d->tile), stream, *d, t, flags);
+    const dim3 grid(d->ntiles), block(NTHREAD);
+    const size_t lds = bsim_step_lds_bytes(d->tile);
+    switch (d->tile / (NWAVE * WAVE)) {
+    case 1: hipLaunchKernelGGL(bsim_step_kernel<1>, grid, block, lds, stream, *d, t, flags); break;
+    case 2: hipLaunchKernelGGL(bsim_step_kernel<2>, grid, block, lds, stream, *d, t, flags); break;
+    case 3: hipLaunchKernelGGL(bsim_step_kernel<3>, grid, block, lds, stream, *d, t, flags); break;
+    case 4: hipLaunchKernelGGL(bsim_step_kernel<4>, grid, block, lds, stream, *d, t, flags); break;
+    case 5: hipLaunchKernelGGL(bsim_step_kernel<5>, grid, block, lds, stream, *d, t, flags); break;
+    case 6: hipLaunchKernelGGL(bsim_step_kernel<6>, grid, block, lds, stream, *d, t, flags); break;
+    case 7: hipLaunchKernelGGL(bsim_step_kernel<7>, grid, block, lds, stream, *d, t, flags); break;
+    case 8: hipLaunchKernelGGL(bsim_step_kernel<8>, grid, block, lds, stream, *d, t, flags); break;
+    default: return hipErrorInvalidValue;
+    }
     return hipGetLastError();
 }
