@@ -2,6 +2,8 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c0]
                     [--cpu-steps C] [--no-cpu]
+    python bench.py --copy-probe              (calibration copy kernel for PMC runs)
+    python bench.py --pmc-summary DIR [--kernel REGEX]
 
 A "step" is one simulation cycle of the whole network (the reference's
 SGSim loop body: LIF update, spike compaction, delayed delivery, synaptic
@@ -24,13 +26,23 @@ two stream synchronisations.  One JSON line is printed:
 Rasters are not recorded in the timed region (recording costs one 8-byte
 store per 64 neurons per cycle); bit-exactness is established by the gpu
 tests on the same code path.
+
+roofline.traffic is the step kernel's HBM bytes per launch, measured by two
+rocprofv3 counter runs (FETCH_SIZE, then WRITE_SIZE) of this script, each
+scaled by what the same counter reads for --copy-probe, a plain copy kernel of
+known size; --pmc-summary prints the per-dispatch averages of such a run.  The
+figures are read from profiles/step_kernel_<workload>_traffic.txt when that
+file exists for the same workload and tile (null otherwise).
 """
 from __future__ import annotations
 
 import argparse
+import csv
 import ctypes as C
+import glob
 import json
 import os
+import re
 import platform
 import sys
 import time
@@ -129,6 +141,67 @@ def cpu_baseline(spec, warmup, steps, threads):
     return r.events_from / r.wall_s, r.events_from, r.wall_s
 
 
+COPY_PROBE_BYTES = 1 << 30
+
+
+def copy_probe():
+    """A plain device copy of known size: 1 GiB read + 1 GiB written, 5 times."""
+    import torch
+    n = COPY_PROBE_BYTES // 4
+    src = torch.rand(n, device="cuda", dtype=torch.float32)
+    dst = torch.empty_like(src)
+    torch.cuda.synchronize()
+    for _ in range(5):
+        torch.mul(src, 1.0, out=dst)
+    torch.cuda.synchronize()
+    print(json.dumps({"copy_probe_bytes_read": COPY_PROBE_BYTES,
+                      "copy_probe_bytes_written": COPY_PROBE_BYTES, "dispatches": 5}))
+
+
+def pmc_summary(path, kernel_re, last=0):
+    """Per-dispatch mean of every counter in rocprofv3 counter_collection.csv
+    files under path, over the dispatches whose kernel name matches (the last
+    `last` of them by dispatch id, if last > 0)."""
+    rx = re.compile(kernel_re)
+    vals = {}
+    files = glob.glob(os.path.join(path, "**", "*counter_collection.csv"), recursive=True)
+    for fn in files:
+        with open(fn, newline="") as f:
+            for row in csv.DictReader(f):
+                if not rx.search(row.get("Kernel_Name", "")):
+                    continue
+                key = (fn, int(row.get("Dispatch_Id", 0)))
+                name = row["Counter_Name"]
+                vals.setdefault(name, {}).setdefault(key, 0.0)
+                vals[name][key] += float(row["Counter_Value"])
+    out = {}
+    for name, v in vals.items():
+        keys = sorted(v, key=lambda k: k[1])
+        if last > 0:
+            keys = keys[-last:]
+        out[name] = {"dispatches": len(keys),
+                     "mean_per_dispatch": sum(v[k] for k in keys) / max(len(keys), 1)}
+    print(json.dumps({"files": len(files), "kernel": kernel_re, "last": last, "counters": out}))
+
+
+def read_traffic(workload, tile):
+    """HBM bytes per launch of the step kernel from the committed counter summary."""
+    fn = os.path.join(ROOT, "profiles", f"step_kernel_{workload}_traffic.txt")
+    if not os.path.exists(fn):
+        return None
+    kv = {}
+    for line in open(fn):
+        if ":" in line and not line.startswith("#"):
+            k, v = line.split(":", 1)
+            kv[k.strip()] = v.strip()
+    try:
+        if int(kv["tile"]) != tile:
+            return None
+        return float(kv["hbm_bytes_per_launch"])
+    except (KeyError, ValueError):
+        return None
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -140,7 +213,17 @@ def main():
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-steps", type=int, default=200)
+    ap.add_argument("--copy-probe", action="store_true")
+    ap.add_argument("--pmc-summary", default=None)
+    ap.add_argument("--kernel", default="bsim_step_kernel")
+    ap.add_argument("--last", type=int, default=0)
     a = ap.parse_args()
+    if a.copy_probe:
+        copy_probe()
+        return
+    if a.pmc_summary:
+        pmc_summary(a.pmc_summary, a.kernel, a.last)
+        return
     if a.gpus != 1:
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU spike "
               "exchange is not part of this build.", file=sys.stderr)
@@ -223,7 +306,8 @@ def main():
                    "synapses": n_syn, "dt_s": netspec.DT, "tile": info["tile"],
                    "simulated_s": a.steps * netspec.DT, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
-                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS, "traffic": None,
+                     "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
+                     "traffic": read_traffic(a.workload, info["tile"]),
                      "kernel": "bsim_step_kernel", "mean_launch_us": mean_ms * 1e3,
                      "events_per_launch": ev_per_launch},
         "cpu_baseline": cpu,

@@ -15,6 +15,7 @@
 #include "internal.h"
 #include "network.h"
 
+extern "C" hipError_t bsim_launch_sched(const bsim_dev *d, hipStream_t stream);
 extern "C" hipError_t bsim_launch_step(const bsim_dev *d, int32_t t, int32_t flags,
                                        hipStream_t stream);
 
@@ -35,6 +36,8 @@ struct bsim_sim {
     int32_t cycle = 0;
     int32_t finished_at = -1;
     int64_t *ev_fin = nullptr;   /* deliveries of the deliver-only launch */
+    hipGraph_t graph = nullptr;  /* kGraphCycles clocked launches */
+    hipGraphExec_t gexec = nullptr;
     int32_t rec_steps = 0, n_rec = 0;
     int64_t n_src = 0, n_off = 0, s = 0;
 };
@@ -211,6 +214,8 @@ static void sim_free(bsim_sim *sim)
     if (!sim) return;
     hipSetDevice(sim->device);
     if (sim->stream) hipStreamSynchronize(sim->stream);
+    if (sim->gexec) hipGraphExecDestroy(sim->gexec);
+    if (sim->graph) hipGraphDestroy(sim->graph);
     for (void *p : sim->allocs) hipFree(p);
     if (sim->stream) hipStreamDestroy(sim->stream);
     delete sim;
@@ -344,6 +349,16 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
         if (!(d.active = dev_alloc<uint64_t>(sim, 2 * nw, e))) goto hip_fail;
         if (!(d.ev_count = dev_alloc<int64_t>(sim, nt, e))) goto hip_fail;
         if (!(sim->ev_fin = dev_alloc<int64_t>(sim, nt, e))) goto hip_fail;
+        if (!(d.clock = dev_alloc<int32_t>(sim, 2, e))) goto hip_fail;
+        hipMemsetAsync(d.clock, 0, 2 * sizeof(int32_t), sim->stream);
+        {
+            std::vector<int32_t> ident(nt);
+            for (size_t i = 0; i < nt; i++) ident[i] = (int32_t)i;
+            d.perm = dev_upload(sim, ident, e);
+            if (e != hipSuccess) goto hip_fail;
+        }
+        if (!(d.tload = dev_alloc<uint32_t>(sim, nt, e))) goto hip_fail;
+        hipMemsetAsync(d.tload, 0, nt * sizeof(uint32_t), sim->stream);
         hipMemsetAsync(d.ie, 0, n * 4, sim->stream);
         hipMemsetAsync(d.ii, 0, n * 4, sim->stream);
         hipMemsetAsync(d.refrac, 0, n * 4, sim->stream);
@@ -382,16 +397,66 @@ hip_fail:
     return nullptr;
 }
 
+}  // extern "C"
+
+/* Cycles per captured graph: runs of this many cycles are replayed from one
+ * hipGraph of identical clocked launches (the kernel reads and advances the
+ * device-side cycle counter), followed by the tile scheduler, which reorders
+ * the tiles by the durations measured in the last cycle. */
+static const int32_t kGraphCycles = 64;
+
+static hipError_t build_graph(bsim_sim *sim)
+{
+    hipError_t e = hipStreamBeginCapture(sim->stream, hipStreamCaptureModeThreadLocal);
+    if (e != hipSuccess) return e;
+    hipError_t le = hipSuccess;
+    for (int32_t i = 0; i < kGraphCycles && le == hipSuccess; i++)
+        le = bsim_launch_step(&sim->d, -1, 0, sim->stream);
+    if (le == hipSuccess) le = bsim_launch_sched(&sim->d, sim->stream);
+    hipGraph_t g = nullptr;
+    e = hipStreamEndCapture(sim->stream, &g);
+    if (le != hipSuccess) e = le;
+    if (e != hipSuccess) {
+        if (g) hipGraphDestroy(g);
+        return e;
+    }
+    hipGraphExec_t x = nullptr;
+    e = hipGraphInstantiate(&x, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        hipGraphDestroy(g);
+        return e;
+    }
+    sim->graph = g;
+    sim->gexec = x;
+    return hipSuccess;
+}
+
+extern "C" {
+
 int32_t bsim_sim_step(bsim_sim *sim, int32_t n)
 {
     if (!sim || n < 0) return fail(BSIM_E_ARG, "bsim_sim_step: bad argument");
+    if ((int64_t)sim->cycle + n > 0x7FFFFFFF) return fail(BSIM_E_ARG, "bsim_sim_step: cycle overflow");
     hipSetDevice(sim->device);
-    for (int32_t i = 0; i < n; i++) {
-        const int32_t t = sim->cycle;
-        const int32_t flags = (t > 0 ? 1 : 0) | 2;
-        hipError_t e = bsim_launch_step(&sim->d, t, flags, sim->stream);
+    hipError_t e = hipSuccess;
+    if (n >= kGraphCycles && !sim->gexec) {
+        e = build_graph(sim);
+        if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("step graph: ") + hipGetErrorString(e));
+    }
+    while (n >= kGraphCycles) {
+        e = hipGraphLaunch(sim->gexec, sim->stream);
+        if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("step graph launch: ") + hipGetErrorString(e));
+        sim->cycle += kGraphCycles;
+        n -= kGraphCycles;
+    }
+    for (; n > 0; n--) {
+        e = bsim_launch_step(&sim->d, -1, 0, sim->stream);
         if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("step launch: ") + hipGetErrorString(e));
         sim->cycle++;
+        if (sim->cycle % kGraphCycles == 0) {
+            e = bsim_launch_sched(&sim->d, sim->stream);
+            if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("sched launch: ") + hipGetErrorString(e));
+        }
     }
     return 0;
 }

@@ -76,6 +76,9 @@ typedef struct bsim_dev {
     const uint32_t *off_tab;    /* BSIM_NWAVE sub-row starts per (pre, delay) */
     /* accounting / recording */
     int64_t *ev_count;          /* [ntiles] deliveries performed */
+    int32_t *clock;             /* [2]: cycle of the next launch, workgroups finished */
+    const int32_t *perm;        /* [ntiles] tile of each workgroup, heaviest first */
+    uint32_t *tload;            /* [ntiles] last clocked launch: duration of each tile */
     uint64_t *raster;           /* [rec_steps][nwords] fired bitmaps, or NULL */
     int32_t rec_steps;
     int32_t rec_t0;             /* cycle of raster/trace row 0 */

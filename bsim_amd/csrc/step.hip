@@ -388,13 +388,23 @@ __device__ void flush_items(const Shared &S, const bsim_dev &d, int32_t nitems, 
     }
 }
 
+/* t >= 0: cycle t with the given flags (1: phase (ii), 2: phase (i)).
+ * t < 0: the cycle is read from d.clock[0] and both phases run (phase (ii)
+ * from cycle 1 on); the last workgroup to finish advances d.clock, so that
+ * identical launches -- a captured graph -- step consecutive cycles. */
 template <int S_>
 __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t t, int32_t flags)
 {
     extern __shared__ __align__(16) uint32_t lds[];
+    const bool clocked = t < 0;
+    if (clocked) {
+        t = d.clock[0];
+        flags = (t > 0 ? 1 : 0) | 2;
+    }
     const int32_t T = d.tile;
-    const int32_t k = blockIdx.x;
+    const int32_t k = d.perm[blockIdx.x];
     const uint32_t tid = threadIdx.x;
+    const uint64_t t_begin = wall_clock64();
     const uint32_t lane = tid & (WAVE - 1);
     const uint32_t w = tid / WAVE;
     const uint32_t qbase = w * ((uint32_t)T / NWAVE);
@@ -532,6 +542,64 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
             }
         }
     }
+    if (clocked) {
+        __syncthreads();            /* every wavefront has read d.clock[0] */
+        if (tid == 0) {
+            const uint64_t dt = wall_clock64() - t_begin;
+            d.tload[k] = dt > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)dt;
+            const int32_t done = atomicAdd(&d.clock[1], 1);
+            if (done == d.ntiles - 1) {
+                d.clock[1] = 0;
+                d.clock[0] = t + 1;
+            }
+        }
+    }
+}
+
+/* Orders the tiles by their measured duration, longest first, so that the
+ * hardware dispatches the heavy tiles before the light ones (results do not
+ * depend on the order).  One workgroup; a counting sort over 64 log-spaced
+ * duration classes (half octaves); within a class the order is arbitrary. */
+#define SCHED_THREADS 1024
+#define SCHED_CLASSES 64
+
+__device__ __forceinline__ uint32_t load_class(uint32_t x)
+{
+    if (x < 2) return 0;
+    const uint32_t e = 31u - (uint32_t)__builtin_clz(x);          /* floor(log2 x) >= 1 */
+    const uint32_t h = (x >> (e - 1)) & 1u;                        /* next bit */
+    return min(2u * e + h, (uint32_t)SCHED_CLASSES - 1u);
+}
+
+__global__ __launch_bounds__(SCHED_THREADS) void bsim_sched_kernel(const uint32_t *tload, int32_t *perm,
+                                                                    int32_t ntiles)
+{
+    __shared__ uint32_t cnt[SCHED_CLASSES], fill[SCHED_CLASSES];
+    const uint32_t tid = threadIdx.x;
+    if (tid < SCHED_CLASSES) cnt[tid] = 0u;
+    __syncthreads();
+    for (int32_t k = tid; k < ntiles; k += SCHED_THREADS) atomicAdd(&cnt[load_class(tload[k])], 1u);
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t s = 0;
+        for (int c = SCHED_CLASSES - 1; c >= 0; c--) {   /* longest class first */
+            fill[c] = s;
+            s += cnt[c];
+        }
+    }
+    __syncthreads();
+    for (int32_t k = tid; k < ntiles; k += SCHED_THREADS) {
+        const uint32_t pos = atomicAdd(&fill[load_class(tload[k])], 1u);
+        perm[pos] = k;
+    }
+}
+
+extern "C" hipError_t bsim_launch_sched(const bsim_dev *d, hipStream_t stream)
+{
+    if (d->ntiles <= 0) return hipSuccess;
+    hipLaunchKernelGGL(bsim_sched_kernel, dim3(1), dim3(SCHED_THREADS), 0, stream, d->tload,
+                       const_cast<int32_t *>(d->perm), d->ntiles);
+    return hipGetLastError();
 }
 
 extern "C" size_t bsim_step_lds_bytes(int32_t tile)

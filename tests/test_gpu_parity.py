@@ -140,6 +140,34 @@ def test_step_in_chunks_equals_one_call():
     assert np.array_equal(s1.trace().view(np.uint32), s2.trace().view(np.uint32))
 
 
+def test_graph_replays_match_single_launches():
+    """Runs of 64 cycles are replayed from a captured graph; single-cycle calls
+    launch one kernel each.  Graph replays interleaved with single launches,
+    host reads and synchronisations give the oracle's result bit for bit, and
+    the run after a read continues where it stopped."""
+    spec = netspec.standard(npop=3, n=400, fanout=60, w0=3.0, drive_rate_hz=400.0, n_drive=100,
+                            drive_w=12.0, seed=11, i_offset=(11.0, 14.0))
+    steps = 64 * 3 + 37
+    o = oracle.run(spec, DT, steps, rec_ids=[5, 700])
+    net = api.Network.from_spec(spec)
+    s1 = api.Simulation(net, DT, rec_steps=steps, rec_ids=[5, 700])
+    done = 0
+    for n in (64, 1, 128, 3, 33):      # graph, eager, 2 graphs, eager, eager
+        s1.step(n)
+        done += n
+        s1.sync()
+        c, ids, _ = s1.raster()
+        off = int(o.raster_count[:done].sum())
+        np.testing.assert_array_equal(c[:done], o.raster_count[:done])
+        np.testing.assert_array_equal(ids, o.raster_ids[:off])
+    assert done == steps
+    s1.finish()
+    s1.sync()
+    assert s1.events() == o.events
+    assert np.array_equal(s1.trace().view(np.uint32), o.rec_v.view(np.uint32))
+    s1.close()
+
+
 def dense_net(n_src=700, n_tgt=300, p_fire=0.9, dmax=3, seed=4):
     """All-to-all from busy Bernoulli sources, every (pre, post) pair created
     twice in a row: ~600 active pres per cycle, rows of 2 x 256 events, so
