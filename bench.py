@@ -17,8 +17,10 @@ two stream synchronisations.  One JSON line is printed:
   roofline     the step kernel's algorithmic bytes (8 B per delivered event:
                fp32 weight + int32 target, as the reference stores them) per
                launch over its mean launch duration, measured with HIP events
-               recorded around each launch on the simulation's stream during a
-               further 200-cycle pass; peak = 8 TB/s (MI355X HBM3E)
+               recorded by the library around each launch on the simulation's
+               stream during a further 200-cycle pass; peak = 8 TB/s (MI355X
+               HBM3E).  The cycle's second kernel (active-row lists) is timed
+               the same way and reported beside it.
   cpu_baseline the oracle's event-driven CSR form (oracle/stsim.c) on the
                host's cores, timed over a window of the same network after
                the same warm-up
@@ -38,7 +40,6 @@ from __future__ import annotations
 
 import argparse
 import csv
-import ctypes as C
 import glob
 import json
 import os
@@ -56,49 +57,6 @@ from bsim_amd import api, netspec  # noqa: E402
 
 HBM_PEAK_GBS = 8000.0
 BYTES_PER_EVENT = 8
-
-
-class HipEvents:
-    """hipEvent timing on an arbitrary stream (ctypes on the HIP runtime)."""
-
-    def __init__(self):
-        for name in ("libamdhip64.so", "libamdhip64.so.7", "/opt/rocm/lib/libamdhip64.so"):
-            try:
-                self.h = C.CDLL(name)
-                break
-            except OSError:
-                continue
-        else:
-            raise RuntimeError("libamdhip64 not found")
-        self.h.hipEventCreate.argtypes = [C.POINTER(C.c_void_p)]
-        self.h.hipEventRecord.argtypes = [C.c_void_p, C.c_void_p]
-        self.h.hipEventSynchronize.argtypes = [C.c_void_p]
-        self.h.hipEventElapsedTime.argtypes = [C.POINTER(C.c_float), C.c_void_p, C.c_void_p]
-        self.h.hipEventDestroy.argtypes = [C.c_void_p]
-
-    def create(self, n):
-        evs = []
-        for _ in range(n):
-            e = C.c_void_p()
-            if self.h.hipEventCreate(C.byref(e)) != 0:
-                raise RuntimeError("hipEventCreate failed")
-            evs.append(e)
-        return evs
-
-    def record(self, e, stream):
-        if self.h.hipEventRecord(e, C.c_void_p(stream)) != 0:
-            raise RuntimeError("hipEventRecord failed")
-
-    def ms(self, a, b):
-        self.h.hipEventSynchronize(b)
-        f = C.c_float()
-        if self.h.hipEventElapsedTime(C.byref(f), a, b) != 0:
-            raise RuntimeError("hipEventElapsedTime failed")
-        return float(f.value)
-
-    def destroy(self, evs):
-        for e in evs:
-            self.h.hipEventDestroy(e)
 
 
 def host_info():
@@ -257,25 +215,17 @@ def main():
     print(f"[bench] timed {a.steps} cycles in {wall:.4f}s, {events} events", file=sys.stderr,
           flush=True)
 
-    # live kernel timing: an event pair around every launch of a further pass
-    hip = HipEvents()
-    stream = sim.stream()
+    # live kernel timing: HIP events around each kernel of a further pass,
+    # recorded by the library on the simulation's stream
     R = a.roofline_steps
-    evs = hip.create(2 * R)
     evr0 = sim.events()
-    for i in range(R):
-        hip.record(evs[2 * i], stream)
-        sim.step(1)
-        hip.record(evs[2 * i + 1], stream)
-    sim.sync()
+    step_us, rows_us = sim.time_steps(R)
     evr1 = sim.events()
-    durs = np.array([hip.ms(evs[2 * i], evs[2 * i + 1]) for i in range(R)])
-    hip.destroy(evs)
-    mean_ms = float(durs.mean())
+    mean_ms = step_us * 1e-3
     ev_per_launch = (evr1 - evr0) / R
     achieved = ev_per_launch * BYTES_PER_EVENT / (mean_ms * 1e-3) / 1e9
-    print(f"[bench] kernel mean {mean_ms * 1e3:.2f} us over {R} launches, "
-          f"{ev_per_launch:.0f} events/launch", file=sys.stderr, flush=True)
+    print(f"[bench] step kernel mean {step_us:.2f} us, row-list kernel mean {rows_us:.2f} us "
+          f"over {R} launches, {ev_per_launch:.0f} events/launch", file=sys.stderr, flush=True)
     sim.close()
 
     cpu = None
@@ -308,8 +258,9 @@ def main():
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
                      "traffic": read_traffic(a.workload, info["tile"]),
-                     "kernel": "bsim_step_kernel", "mean_launch_us": mean_ms * 1e3,
-                     "events_per_launch": ev_per_launch},
+                     "kernel": "bsim_step_kernel", "mean_launch_us": step_us,
+                     "events_per_launch": ev_per_launch,
+                     "row_list_kernel_us": rows_us},
         "cpu_baseline": cpu,
     }
     print(json.dumps(out), flush=True)

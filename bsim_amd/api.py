@@ -48,6 +48,7 @@ _SIGS = {
     "bsim_sim_read_trace": (C.c_int32, [P, P]),
     "bsim_sim_read_state": (C.c_int32, [P, P, P, P, P, P]),
     "bsim_sim_events": (C.c_int64, [P]),
+    "bsim_sim_time_steps": (C.c_int32, [P, C.c_int32, P, P]),
     "bsim_sim_get_info": (C.c_int32, [P, P]),
     "bsim_sim_stream": (P, [P]),
     "bsim_sim_destroy": (None, [P]),
@@ -347,6 +348,15 @@ class Simulation:
         if r < 0:
             _err(r, "bsim_sim_read_state")
         return dict(v=v, ie=ie, ii=ii, refrac=rf, fire_count=fc)
+
+    def time_steps(self, n):
+        """Advances n cycles one launch at a time with HIP events around each
+        kernel; returns the mean (step kernel, row-list kernel) durations in us."""
+        a, b = C.c_double(), C.c_double()
+        r = lib().bsim_sim_time_steps(self._h, int(n), C.byref(a), C.byref(b))
+        if r < 0:
+            _err(r, "bsim_sim_time_steps")
+        return a.value, b.value
 
     def events(self):
         r = lib().bsim_sim_events(self._h)

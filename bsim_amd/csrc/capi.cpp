@@ -16,6 +16,7 @@
 #include "network.h"
 
 extern "C" hipError_t bsim_launch_sched(const bsim_dev *d, hipStream_t stream);
+extern "C" hipError_t bsim_launch_rows(const bsim_dev *d, hipStream_t stream);
 extern "C" hipError_t bsim_launch_step(const bsim_dev *d, int32_t t, int32_t flags,
                                        hipStream_t stream);
 
@@ -333,6 +334,10 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
     UP(tile_rng, L.tile_rng);
     UP(rng, L.rng);
     UP(off_tab, L.off_tab);
+    UP(src_lo, L.src_lo);
+    UP(src_hi, L.src_hi);
+    UP(src_list_base, L.src_list_base);
+    d.nsrc = (int32_t)L.src_lo.size();
     /* state */
     UP(v, L.v_init);
     UP(arr_cur, L.arr_cur0);
@@ -359,6 +364,11 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
         }
         if (!(d.tload = dev_alloc<uint32_t>(sim, nt, e))) goto hip_fail;
         hipMemsetAsync(d.tload, 0, nt * sizeof(uint32_t), sim->stream);
+        const size_t ns = (size_t)(d.nsrc > 0 ? d.nsrc : 1);
+        if (!(d.src_count = dev_alloc<uint32_t>(sim, ns, e))) goto hip_fail;
+        hipMemsetAsync(d.src_count, 0, ns * sizeof(uint32_t), sim->stream);
+        if (!(d.src_list = dev_alloc<uint32_t>(sim, (size_t)(L.src_list_len > 0 ? L.src_list_len : 1), e)))
+            goto hip_fail;
         hipMemsetAsync(d.ie, 0, n * 4, sim->stream);
         hipMemsetAsync(d.ii, 0, n * 4, sim->stream);
         hipMemsetAsync(d.refrac, 0, n * 4, sim->stream);
@@ -410,8 +420,10 @@ static hipError_t build_graph(bsim_sim *sim)
     hipError_t e = hipStreamBeginCapture(sim->stream, hipStreamCaptureModeThreadLocal);
     if (e != hipSuccess) return e;
     hipError_t le = hipSuccess;
-    for (int32_t i = 0; i < kGraphCycles && le == hipSuccess; i++)
+    for (int32_t i = 0; i < kGraphCycles && le == hipSuccess; i++) {
         le = bsim_launch_step(&sim->d, -1, 0, sim->stream);
+        if (le == hipSuccess) le = bsim_launch_rows(&sim->d, sim->stream);
+    }
     if (le == hipSuccess) le = bsim_launch_sched(&sim->d, sim->stream);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(sim->stream, &g);
@@ -451,6 +463,7 @@ int32_t bsim_sim_step(bsim_sim *sim, int32_t n)
     }
     for (; n > 0; n--) {
         e = bsim_launch_step(&sim->d, -1, 0, sim->stream);
+        if (e == hipSuccess) e = bsim_launch_rows(&sim->d, sim->stream);
         if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("step launch: ") + hipGetErrorString(e));
         sim->cycle++;
         if (sim->cycle % kGraphCycles == 0) {
@@ -458,6 +471,41 @@ int32_t bsim_sim_step(bsim_sim *sim, int32_t n)
             if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("sched launch: ") + hipGetErrorString(e));
         }
     }
+    return 0;
+}
+
+int32_t bsim_sim_time_steps(bsim_sim *sim, int32_t n, double *step_us, double *rows_us)
+{
+    if (!sim || n <= 0) return fail(BSIM_E_ARG, "bsim_sim_time_steps: bad argument");
+    hipSetDevice(sim->device);
+    std::vector<hipEvent_t> ev((size_t)n * 3, nullptr);
+    hipError_t e = hipSuccess;
+    for (size_t i = 0; i < ev.size() && e == hipSuccess; i++) e = hipEventCreate(&ev[i]);
+    for (int32_t i = 0; i < n && e == hipSuccess; i++) {
+        e = hipEventRecord(ev[3 * i], sim->stream);
+        if (e == hipSuccess) e = bsim_launch_step(&sim->d, -1, 0, sim->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[3 * i + 1], sim->stream);
+        if (e == hipSuccess) e = bsim_launch_rows(&sim->d, sim->stream);
+        if (e == hipSuccess) e = hipEventRecord(ev[3 * i + 2], sim->stream);
+        if (e == hipSuccess) {
+            sim->cycle++;
+            if (sim->cycle % kGraphCycles == 0) e = bsim_launch_sched(&sim->d, sim->stream);
+        }
+    }
+    if (e == hipSuccess) e = hipStreamSynchronize(sim->stream);
+    double a = 0.0, b = 0.0;
+    for (int32_t i = 0; i < n && e == hipSuccess; i++) {
+        float x = 0.f, y = 0.f;
+        e = hipEventElapsedTime(&x, ev[3 * i], ev[3 * i + 1]);
+        if (e == hipSuccess) e = hipEventElapsedTime(&y, ev[3 * i + 1], ev[3 * i + 2]);
+        a += x;
+        b += y;
+    }
+    for (hipEvent_t x : ev)
+        if (x) hipEventDestroy(x);
+    if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("time_steps: ") + hipGetErrorString(e));
+    if (step_us) *step_us = a * 1e3 / n;
+    if (rows_us) *rows_us = b * 1e3 / n;
     return 0;
 }
 

@@ -40,9 +40,14 @@
 enum { BSIM_KIND_LIF = 0, BSIM_KIND_ARRAY = 1, BSIM_KIND_BERNOULLI = 2 };
 
 typedef struct bsim_range {
-    int32_t lo, hi;     /* pre neurons [lo, hi) with synapses into the tile */
-    int64_t off_base;   /* first index into off_tab */
+    int32_t lo, hi;     /* pre neurons [lo, hi) of one chunk of a source range of the tile */
+    int64_t off_base;   /* index into off_tab of row (lo, 1) */
+    int32_t uid;        /* source chunk: index into src_* */
+    int32_t pad;
 } bsim_range;
+
+#define BSIM_MAX_RANGE (1 << 26)   /* pres per source range: row index fits 32 bits */
+#define BSIM_CHUNK 4096             /* pres per source chunk (one row-list workgroup) */
 
 /* Kernel arguments (passed by value). */
 typedef struct bsim_dev {
@@ -74,6 +79,18 @@ typedef struct bsim_dev {
     const int32_t *tile_rng;    /* [ntiles + 1] first source range of each tile */
     const bsim_range *rng;
     const uint32_t *off_tab;    /* BSIM_NWAVE sub-row starts per (pre, delay) */
+    /* source chunks and their active-row lists: the distinct source ranges of
+     * all tiles, cut into chunks of at most BSIM_CHUNK pres (a tile's range
+     * entries are per chunk).  After the update of cycle t, list u holds the
+     * row index (p - lo) * D + d - 1 of every (pre, delay) of chunk u that
+     * delivers in phase (ii) of cycle t, ascending; it is shared by every
+     * tile that receives from the chunk */
+    int32_t nsrc;
+    int32_t pad2;
+    const int32_t *src_lo, *src_hi;
+    const int64_t *src_list_base;
+    uint32_t *src_list;
+    uint32_t *src_count;
     /* accounting / recording */
     int64_t *ev_count;          /* [ntiles] deliveries performed */
     int32_t *clock;             /* [2]: cycle of the next launch, workgroups finished */

@@ -9,6 +9,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <map>
 #include <cstring>
 #include <numeric>
 
@@ -236,7 +237,7 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
             int32_t plast = p0;
             while (f < m) {
                 const int32_t pf = (int32_t)(ent[f].key >> 8);
-                if (pf - plast > kRangeGap) break;
+                if (pf - plast > kRangeGap || pf - p0 >= BSIM_MAX_RANGE) break;
                 plast = pf;
                 f++;
             }
@@ -244,6 +245,8 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
             r.lo = p0;
             r.hi = plast + 1;
             r.off_base = (int64_t)toff[t].size();
+            r.uid = 0;
+            r.pad = 0;
             const int64_t cnt = (int64_t)(r.hi - r.lo) * D * NW + 1;
             const size_t base = toff[t].size();
             toff[t].resize(base + cnt);
@@ -270,22 +273,46 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
     }
     size_t nr = 0, noff = 0;
     for (int32_t t = 0; t < NT; t++) {
-        L.tile_rng[t + 1] = L.tile_rng[t] + (int32_t)trng[t].size();
         nr += trng[t].size();
         noff += toff[t].size();
     }
     L.rng.reserve(nr);
     L.off_tab.reserve(noff);
+    /* distinct source ranges, each cut into chunks of BSIM_CHUNK pres; a
+     * tile's range becomes one entry per chunk */
+    std::map<std::pair<int32_t, int32_t>, int32_t> uid;
+    std::vector<int32_t> tile_nrng(NT, 0);
     for (int32_t t = 0; t < NT; t++) {
         const int64_t ob = (int64_t)L.off_tab.size();
-        for (auto r : trng[t]) {
-            r.off_base += ob;
-            L.rng.push_back(r);
+        for (const auto &r : trng[t]) {
+            auto it = uid.find({r.lo, r.hi});
+            if (it == uid.end()) {
+                it = uid.emplace(std::make_pair(r.lo, r.hi), (int32_t)L.src_lo.size()).first;
+                for (int32_t c = r.lo; c < r.hi; c += BSIM_CHUNK) {
+                    const int32_t ce = std::min(r.hi, c + BSIM_CHUNK);
+                    L.src_lo.push_back(c);
+                    L.src_hi.push_back(ce);
+                    L.src_list_base.push_back(L.src_list_len);
+                    L.src_list_len += (int64_t)(ce - c) * D;
+                }
+            }
+            int32_t u = it->second;
+            for (int32_t c = r.lo; c < r.hi; c += BSIM_CHUNK, u++) {
+                bsim_range rc;
+                rc.lo = c;
+                rc.hi = std::min(r.hi, c + BSIM_CHUNK);
+                rc.off_base = ob + r.off_base + (int64_t)(c - r.lo) * D * NW;
+                rc.uid = u;
+                rc.pad = 0;
+                L.rng.push_back(rc);
+                tile_nrng[t]++;
+            }
         }
         L.off_tab.insert(L.off_tab.end(), toff[t].begin(), toff[t].end());
         std::vector<bsim_range>().swap(trng[t]);
         std::vector<uint32_t>().swap(toff[t]);
     }
+    for (int32_t t = 0; t < NT; t++) L.tile_rng[t + 1] = L.tile_rng[t] + tile_nrng[t];
     return 0;
 }
 

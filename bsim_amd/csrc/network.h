@@ -57,6 +57,9 @@ struct Layout {
     std::vector<int32_t> tile_rng;
     std::vector<bsim_range> rng;
     std::vector<uint32_t> off_tab;
+    std::vector<int32_t> src_lo, src_hi;     /* distinct source ranges */
+    std::vector<int64_t> src_list_base;      /* list capacity (hi - lo) * D each */
+    int64_t src_list_len = 0;
 };
 
 /* LIFENeuron::reset (src/neuron/lif/LIFENeuron.cpp:123-167) for one neuron,

@@ -48,16 +48,20 @@
 #include "layout.h"
 
 #define WAVE 64
-#define NWAVE BSIM_NWAVE
-#define NTHREAD (WAVE * NWAVE)
-#define ITEMCAP 512      /* items staged per pass: 2 per thread */
-#define RCAP 512         /* rows staged per window: 2 per thread */
-#define ROWW (NWAVE + 1) /* uint32 per staged row: quarter boundaries */
-#define NGRP 4           /* 64-position groups per streaming step */
+#define NWAVE BSIM_NWAVE              /* streaming wavefronts = target quarters */
+#define NTHREAD (WAVE * (NWAVE + 1))  /* plus one producer wavefront */
+#define RCAP 512                      /* rows per staged window */
+#define RPL (RCAP / WAVE)             /* rows per producer lane */
+#define ROWW (NWAVE + 1)              /* uint32 per staged row: quarter boundaries */
+#define NGRP 4                        /* 64-position groups per streaming step */
 #define SSTEP (NGRP * WAVE)
+#define MAXR 128                      /* source ranges per batch */
+#define ROWS_THREADS 1024             /* row-list kernel: 4 pres per thread */
+static_assert(BSIM_CHUNK == 4 * ROWS_THREADS, "a source chunk is one row-list workgroup");
 
 /* LDS bytes of a workgroup */
-#define LDS_FIXED (4 * (RCAP * ROWW + ITEMCAP + NWAVE * SSTEP * 2 + 2 * NWAVE * WAVE + 16 + NWAVE))
+#define LDS_FIXED (4 * (2 * RCAP * ROWW + 2 * NWAVE + NWAVE * SSTEP * 2 + 2 * NWAVE * WAVE) + \
+                   4 * (MAXR + 1) + 4 + 16 * MAXR)
 
 __device__ __forceinline__ uint32_t bern_hash(uint64_t seed, uint32_t id, uint32_t t)
 {
@@ -105,35 +109,15 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x)
     return (uint32_t)v;
 }
 
-/* exclusive prefix sum over the workgroup; *total = sum over all threads.
- * scr: NWAVE uint32 of LDS.  Contains two barriers. */
-__device__ __forceinline__ uint32_t block_excl_scan(uint32_t x, uint32_t *total, uint32_t *scr)
-{
-    const int lane = threadIdx.x & (WAVE - 1);
-    const int w = threadIdx.x / WAVE;
-    const uint32_t v = wave_incl_scan(x);
-    if (lane == WAVE - 1) scr[w] = v;
-    __syncthreads();
-    uint32_t before = 0, tot = 0;
-#pragma unroll
-    for (int i = 0; i < NWAVE; i++) {
-        const uint32_t s = scr[i];
-        before += (i < w) ? s : 0u;
-        tot += s;
-    }
-    __syncthreads();
-    *total = tot;
-    return before + v - x;
-}
-
 struct Shared {
-    uint32_t *rows;      /* [RCAP][ROWW] quarter boundaries of the staged rows */
-    uint32_t *items;     /* [ITEMCAP] pre ids */
+    uint32_t *rows;      /* [2][RCAP][ROWW] quarter boundaries of the staged rows, double-buffered */
+    uint32_t *tot;       /* [2][NWAVE] stream length of each quarter in the window */
     uint2 *stage;        /* [NWAVE][SSTEP] (local target in quarter, weight bits) */
     uint32_t *hflag;     /* [NWAVE][WAVE] row-head flags of a 64-position group */
     uint32_t *hbase;     /* [NWAVE][WAVE] row-head address bases */
-    uint32_t *scr;       /* [16] scratch */
-    uint32_t *tot;       /* [NWAVE] stream length of each quarter in the window */
+    uint32_t *cum;       /* [MAXR + 1] rows before each source range of the batch */
+    int64_t *lbase;      /* [MAXR] first list entry of each source range */
+    int64_t *obase;      /* [MAXR] row-table base of each source range */
 };
 
 /* A window of 64 staged rows in the lanes of a wavefront: lane j holds row
@@ -146,7 +130,7 @@ struct RowWin {
     uint32_t end;    /* stream position after the window (uniform) */
 };
 
-__device__ __forceinline__ void load_win(const Shared &S, RowWin &W, int32_t rw, uint32_t pos0,
+__device__ __forceinline__ void load_win(const uint32_t *rows, RowWin &W, int32_t rw, uint32_t pos0,
                                          int32_t nrows)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
@@ -154,8 +138,8 @@ __device__ __forceinline__ void load_win(const Shared &S, RowWin &W, int32_t rw,
     const int32_t j = rw + (int32_t)lane;
     uint32_t s0 = 0, s1 = 0;
     if (j < nrows) {
-        s0 = S.rows[j * ROWW + q];
-        s1 = S.rows[j * ROWW + q + 1];
+        s0 = rows[j * ROWW + q];
+        s1 = rows[j * ROWW + q + 1];
     }
     W.rw = rw;
     W.len = s1 - s0;
@@ -168,8 +152,8 @@ __device__ __forceinline__ void load_win(const Shared &S, RowWin &W, int32_t rw,
 /* Maps stream positions [b, b + 64) to synapse indices: lane i gets position
  * b + i.  cur is the address base of the row covering position b - 1.
  * Returns false for positions past the end of the stream. */
-__device__ __forceinline__ bool map_group(const Shared &S, RowWin &W, uint32_t b, uint32_t &cur,
-                                          uint32_t &addr, int32_t nrows)
+__device__ __forceinline__ bool map_group(const Shared &S, const uint32_t *rows, RowWin &W, uint32_t b,
+                                          uint32_t &cur, uint32_t &addr, int32_t nrows)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     const uint32_t q = threadIdx.x / WAVE;
@@ -202,7 +186,7 @@ __device__ __forceinline__ bool map_group(const Shared &S, RowWin &W, uint32_t b
         wave_sync();
         if (W.end >= b + WAVE) break;             /* uniform */
         if (W.rw + WAVE >= nrows) break;          /* rows exhausted: stream ends */
-        load_win(S, W, W.rw + WAVE, W.end, nrows);
+        load_win(rows, W, W.rw + WAVE, W.end, nrows);
     }
     cur = readlane(mb, WAVE - 1);
     addr = mb + p;
@@ -219,8 +203,8 @@ struct Step {
 /* The loads are issued unconditionally (lanes without an event read the
  * tile's first synapse), so that the compiler can count them exactly and the
  * apply of the previous step waits only for its own loads. */
-__device__ __forceinline__ void load_step(const Shared &S, RowWin &W, uint32_t b0, uint32_t tot,
-                                          uint32_t &cur, int32_t nrows, uint32_t nsyn,
+__device__ __forceinline__ void load_step(const Shared &S, const uint32_t *rows, RowWin &W, uint32_t b0,
+                                          uint32_t tot, uint32_t &cur, int32_t nrows, uint32_t nsyn,
                                           const uint16_t *tgt, const float *wts, Step &st)
 {
 #pragma unroll
@@ -228,7 +212,7 @@ __device__ __forceinline__ void load_step(const Shared &S, RowWin &W, uint32_t b
         const uint32_t b = b0 + g * WAVE;
         uint32_t addr = 0;
         bool v = false;
-        if (b < tot) v = map_group(S, W, b, cur, addr, nrows);   /* uniform branch */
+        if (b < tot) v = map_group(S, rows, W, b, cur, addr, nrows);   /* uniform branch */
         addr = (v && addr < nsyn) ? addr : 0u;
         st.v[g] = v;
         st.tl[g] = tgt[addr];
@@ -299,99 +283,164 @@ __device__ __forceinline__ void apply_step(const Shared &S, const Step &st, uint
 /* Streams this wavefront's quarter of the staged rows [0, nrows).  The loads
  * of step s+1 are in flight while step s is applied. */
 template <int S_>
-__device__ void stream_rows(const Shared &S, int32_t nrows, uint32_t tot, uint32_t qbase,
-                            uint32_t nsyn, const uint16_t *tgt, const float *wts, Acc<S_> &acc)
+__device__ void stream_rows(const Shared &S, const uint32_t *rows, int32_t nrows, uint32_t tot,
+                            uint32_t qbase, uint32_t nsyn, const uint16_t *tgt, const float *wts,
+                            Acc<S_> &acc)
 {
     if (tot == 0) return;                                     /* uniform */
     RowWin W;
-    load_win(S, W, 0, 0, nrows);
+    load_win(rows, W, 0, 0, nrows);
     uint32_t cur = 0;
     Step A, B;
-    load_step(S, W, 0, tot, cur, nrows, nsyn, tgt, wts, A);
+    load_step(S, rows, W, 0, tot, cur, nrows, nsyn, tgt, wts, A);
     uint32_t b = 0;
     while (true) {
-        load_step(S, W, b + SSTEP, tot, cur, nrows, nsyn, tgt, wts, B);
+        load_step(S, rows, W, b + SSTEP, tot, cur, nrows, nsyn, tgt, wts, B);
         apply_step<S_>(S, A, qbase, acc);
         b += SSTEP;
         if (b >= tot) break;
-        load_step(S, W, b + SSTEP, tot, cur, nrows, nsyn, tgt, wts, A);
+        load_step(S, rows, W, b + SSTEP, tot, cur, nrows, nsyn, tgt, wts, A);
         apply_step<S_>(S, B, qbase, acc);
         b += SSTEP;
         if (b >= tot) break;
     }
 }
 
-/* Turns the staged items [0, nitems) of source range r into rows and streams
- * them.  Items are pre ids, ascending. */
-template <int S_>
-__device__ void flush_items(const Shared &S, const bsim_dev &d, int32_t nitems, const bsim_range &r,
-                            int32_t par, uint32_t qbase, uint32_t nsyn, const uint16_t *tgt,
-                            const float *wts, int64_t &events, Acc<S_> &acc)
+/* ---- the producer wavefront: staging of row windows ---------------------- */
+
+/* Row j of a window of the current range batch belongs to the source range
+ * whose cumulative row interval holds it. */
+__device__ __forceinline__ int32_t range_of(const Shared &S, uint32_t j, int32_t nrb)
 {
-    const uint32_t tid = threadIdx.x;
-    const uint32_t lane = tid & (WAVE - 1);
-    const uint32_t q = tid / WAVE;
-    const int32_t D = d.dmax;
-    const uint64_t dmask = (D >= 63) ? (~0ull >> 1) : ((1ull << D) - 1ull);
-    const uint64_t *hist = d.hist + (int64_t)par * d.n;
-    uint32_t p[2];
-    uint64_t m[2];
-    uint32_t lp[2], cnt = 0;
-#pragma unroll
-    for (int j = 0; j < 2; j++) {
-        const int32_t i = 2 * tid + j;
-        p[j] = i < nitems ? S.items[i] : 0u;
-        m[j] = i < nitems ? ((hist[p[j]] >> 1) & dmask) : 0ull;
+    int32_t a = 0, b = nrb - 1;           /* largest i with cum[i] <= j */
+    while (a < b) {
+        const int32_t m = (a + b + 1) >> 1;
+        if (S.cum[m] <= j) a = m; else b = m - 1;
     }
+    return a;
+}
+
+/* Active-row list entries of window `win` (lane holds rows lane + 64 x). */
+__device__ __forceinline__ void prod_entries(const Shared &S, const bsim_dev &d, int32_t win,
+                                             uint32_t nrows_tot, int32_t nrb, uint32_t (&E)[RPL])
+{
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
 #pragma unroll
-    for (int j = 0; j < 2; j++) {
-        lp[j] = cnt;
-        cnt += (uint32_t)__popcll(m[j]);
+    for (int x = 0; x < RPL; x++) {
+        const uint32_t j = (uint32_t)win * RCAP + lane + WAVE * x;
+        E[x] = 0u;
+        if (j < nrows_tot) {
+            const int32_t ri = range_of(S, j, nrb);
+            E[x] = d.src_list[S.lbase[ri] + (j - S.cum[ri])];
+        }
     }
-    uint32_t tot;
-    const uint32_t ex = block_excl_scan(cnt, &tot, S.scr);
-    for (uint32_t emitted = 0; emitted < tot; emitted += RCAP) {
-        const uint32_t wend = min(tot, emitted + RCAP);
-        uint32_t qlen[NWAVE] = {0, 0, 0, 0};
+}
+
+/* Stages window `win` into LDS buffer `buf`: the row-table entries of its
+ * rows (from the list entries E), and the stream length of each quarter.
+ * Loads and stores go in two halves to bound the registers held. */
+__device__ __forceinline__ void prod_stage(const Shared &S, const bsim_dev &d, int32_t buf, int32_t win,
+                                           uint32_t nrows_tot, int32_t nrb, const uint32_t (&E)[RPL])
+{
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    uint32_t *rows = S.rows + buf * (RCAP * ROWW);
+    uint32_t ql[NWAVE] = {0, 0, 0, 0};
 #pragma unroll
-        for (int j = 0; j < 2; j++) {
-            uint64_t mm = m[j];
-            uint32_t rank = ex + lp[j];
-            while (mm) {
-                const int32_t bit = __builtin_ctzll(mm);
-                mm &= mm - 1;
-                if (rank >= emitted && rank < wend) {
-                    const int64_t e0 = r.off_base +
-                                       ((int64_t)(p[j] - (uint32_t)r.lo) * D + bit) * NWAVE;
-                    uint32_t *row = S.rows + (rank - emitted) * ROWW;
-                    uint32_t o[ROWW];
+    for (int h = 0; h < 2; h++) {
+        uint32_t Tb[RPL / 2][ROWW];
 #pragma unroll
-                    for (int x = 0; x < ROWW; x++) o[x] = d.off_tab[e0 + x];
+        for (int x = 0; x < RPL / 2; x++) {
+            const uint32_t jj = lane + WAVE * (h * (RPL / 2) + x);
+            const uint32_t j = (uint32_t)win * RCAP + jj;
+            int64_t e0 = 0;
+            if (j < nrows_tot) e0 = S.obase[range_of(S, j, nrb)] + (int64_t)E[h * (RPL / 2) + x] * NWAVE;
 #pragma unroll
-                    for (int x = 0; x < ROWW; x++) row[x] = o[x];
-#pragma unroll
-                    for (int x = 0; x < NWAVE; x++) qlen[x] += o[x + 1] - o[x];
-                }
-                rank++;
-            }
+            for (int c = 0; c < ROWW; c++) Tb[x][c] = (j < nrows_tot) ? d.off_tab[e0 + c] : 0u;
         }
 #pragma unroll
-        for (int x = 0; x < NWAVE; x++)
-            if (qlen[x]) atomicAdd(&S.tot[x], qlen[x]);
-        __syncthreads();
-        const uint32_t qtot = S.tot[q];
-        wave_sync();
-        if (lane == 0) S.tot[q] = 0u;       /* read by this wavefront only */
-        events += qtot;
-        stream_rows<S_>(S, (int32_t)(wend - emitted), qtot, qbase, nsyn, tgt, wts, acc);
+        for (int x = 0; x < RPL / 2; x++) {
+            const uint32_t jj = lane + WAVE * (h * (RPL / 2) + x);
+            if ((uint32_t)win * RCAP + jj < nrows_tot) {
+#pragma unroll
+                for (int c = 0; c < ROWW; c++) rows[jj * ROWW + c] = Tb[x][c];
+#pragma unroll
+                for (int q = 0; q < NWAVE; q++) ql[q] += Tb[x][q + 1] - Tb[x][q];
+            }
+        }
+    }
+#pragma unroll
+    for (int q = 0; q < NWAVE; q++) {
+        const uint32_t sum = readlane(wave_incl_scan(ql[q]), WAVE - 1);
+        if (lane == 0) S.tot[buf * NWAVE + q] = sum;
+    }
+}
+
+/* Phase (ii) of one batch of up to MAXR source ranges of the tile: wavefront
+ * NWAVE stages window i+1 of the concatenated active-row lists while
+ * wavefronts 0..NWAVE-1 stream window i. */
+template <int S_>
+__device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, int32_t nrb,
+                              uint32_t qbase, uint32_t nsyn, const uint16_t *tgt, const float *wts,
+                              int64_t &events, Acc<S_> &acc)
+{
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t w = threadIdx.x / WAVE;
+    const bool prod = w == NWAVE;
+    if (prod) {
+        /* range metadata and the cumulative row counts of the batch */
+        uint32_t c[MAXR / WAVE];
+#pragma unroll
+        for (int x = 0; x < MAXR / WAVE; x++) {
+            const int32_t i = (int32_t)lane * (MAXR / WAVE) + x;
+            c[x] = 0u;
+            if (i < nrb) {
+                const bsim_range r = d.rng[r0 + i];
+                c[x] = d.src_count[r.uid];
+                S.lbase[i] = d.src_list_base[r.uid];
+                S.obase[i] = r.off_base;
+            }
+        }
+        uint32_t sum = 0;
+#pragma unroll
+        for (int x = 0; x < MAXR / WAVE; x++) sum += c[x];
+        const uint32_t incl = wave_incl_scan(sum);
+        uint32_t run = incl - sum;
+#pragma unroll
+        for (int x = 0; x < MAXR / WAVE; x++) {
+            const int32_t i = (int32_t)lane * (MAXR / WAVE) + x;
+            if (i < nrb) S.cum[i] = run;
+            run += c[x];
+        }
+        if (lane == WAVE - 1) S.cum[nrb] = incl;
+    }
+    __syncthreads();
+    const uint32_t nrows_tot = S.cum[nrb];
+    const int32_t nwin = (int32_t)((nrows_tot + RCAP - 1) / RCAP);
+    if (nwin == 0) return;                                    /* uniform */
+    uint32_t E[RPL];
+    if (prod) {
+        prod_entries(S, d, 0, nrows_tot, nrb, E);
+        prod_stage(S, d, 0, 0, nrows_tot, nrb, E);
+        if (nwin > 1) prod_entries(S, d, 1, nrows_tot, nrb, E);
+    }
+    __syncthreads();
+    for (int32_t i = 0; i < nwin; i++) {
+        const int32_t buf = i & 1;
+        if (prod) {
+            if (i + 1 < nwin) {
+                prod_stage(S, d, buf ^ 1, i + 1, nrows_tot, nrb, E);
+                if (i + 2 < nwin) prod_entries(S, d, i + 2, nrows_tot, nrb, E);
+            }
+        } else {
+            const int32_t nrows = (int32_t)min((uint32_t)RCAP, nrows_tot - (uint32_t)i * RCAP);
+            const uint32_t qtot = S.tot[buf * NWAVE + w];
+            events += qtot;
+            stream_rows<S_>(S, S.rows + buf * (RCAP * ROWW), nrows, qtot, qbase, nsyn, tgt, wts, acc);
+        }
         __syncthreads();
     }
 }
 
-/* t >= 0: cycle t with the given flags (1: phase (ii), 2: phase (i)).
- * t < 0: the cycle is read from d.clock[0] and both phases run (phase (ii)
- * from cycle 1 on); the last workgroup to finish advances d.clock, so that
- * identical launches -- a captured graph -- step consecutive cycles. */
 template <int S_>
 __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t t, int32_t flags)
 {
@@ -410,14 +459,18 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
     const uint32_t qbase = w * ((uint32_t)T / NWAVE);
 
     Shared S;
-    uint32_t *p = lds;
-    S.stage = reinterpret_cast<uint2 *>(p);           p += NWAVE * SSTEP * 2;
-    S.rows = p;                                       p += RCAP * ROWW;
-    S.items = p;                                      p += ITEMCAP;
-    S.hflag = p;                                      p += NWAVE * WAVE;
-    S.hbase = p;                                      p += NWAVE * WAVE;
-    S.scr = p;                                        p += 16;
-    S.tot = p;
+    {
+        int64_t *p8 = reinterpret_cast<int64_t *>(lds);
+        S.lbase = p8;                                 p8 += MAXR;
+        S.obase = p8;                                 p8 += MAXR;
+        S.stage = reinterpret_cast<uint2 *>(p8);
+        uint32_t *p = reinterpret_cast<uint32_t *>(p8) + NWAVE * SSTEP * 2;
+        S.rows = p;                                   p += 2 * RCAP * ROWW;
+        S.tot = p;                                    p += 2 * NWAVE;
+        S.hflag = p;                                  p += NWAVE * WAVE;
+        S.hbase = p;                                  p += NWAVE * WAVE;
+        S.cum = p;
+    }
 
     Acc<S_> acc;
 #pragma unroll
@@ -428,56 +481,20 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
 
     /* ---- phase (ii) of cycle t-1 ---------------------------------------- */
     if (flags & 1) {
-        if (tid < NWAVE) S.tot[tid] = 0u;
-        __syncthreads();
-        const int32_t par = (t - 1) & 1;
         const int64_t syn0 = d.tile_syn[k];
         const uint32_t nsyn = (uint32_t)(d.tile_syn[k + 1] - syn0);
         const uint16_t *tgt = d.syn_tgt + syn0;
         const float *wts = d.syn_w + syn0;
-        const uint64_t *act = d.active + (int64_t)par * d.nwords;
         int64_t events = 0;
-        for (int32_t ri = d.tile_rng[k]; ri < d.tile_rng[k + 1]; ri++) {
-            const bsim_range r = d.rng[ri];
-            const int32_t wlo = r.lo >> 6, whi = (r.hi + 63) >> 6;
-            int32_t nitems = 0;
-            for (int32_t wb = wlo; wb < whi; wb += NTHREAD) {
-                const int32_t wi = wb + (int32_t)tid;
-                uint64_t word = 0;
-                if (wi < whi) {
-                    word = act[wi];
-                    if (wi == wlo && (r.lo & 63)) word &= ~0ull << (r.lo & 63);
-                    if (wi == whi - 1 && (r.hi & 63)) word &= ~0ull >> (64 - (r.hi & 63));
-                }
-                uint32_t tot;
-                const uint32_t ex = block_excl_scan((uint32_t)__popcll(word), &tot, S.scr);
-                for (uint32_t emitted = 0; emitted < tot;) {
-                    const uint32_t n = min((uint32_t)(ITEMCAP - nitems), tot - emitted);
-                    uint64_t mm = word;
-                    uint32_t rank = ex;
-                    while (mm) {
-                        const int32_t bit = __builtin_ctzll(mm);
-                        mm &= mm - 1;
-                        if (rank >= emitted && rank < emitted + n)
-                            S.items[nitems + (rank - emitted)] = (uint32_t)(wi * 64 + bit);
-                        rank++;
-                    }
-                    __syncthreads();
-                    nitems += (int32_t)n;
-                    emitted += n;
-                    if (nitems == ITEMCAP) {
-                        flush_items<S_>(S, d, nitems, r, par, qbase, nsyn, tgt, wts, events, acc);
-                        nitems = 0;
-                    }
-                }
-            }
-            if (nitems > 0) flush_items<S_>(S, d, nitems, r, par, qbase, nsyn, tgt, wts, events, acc);
-        }
-        if (lane == 0) atomicAdd((unsigned long long *)&d.ev_count[k], (unsigned long long)events);
+        const int32_t r0 = d.tile_rng[k], r1 = d.tile_rng[k + 1];
+        for (int32_t rb = r0; rb < r1; rb += MAXR)
+            deliver_batch<S_>(S, d, rb, min(MAXR, r1 - rb), qbase, nsyn, tgt, wts, events, acc);
+        if (lane == 0 && w < NWAVE)
+            atomicAdd((unsigned long long *)&d.ev_count[k], (unsigned long long)events);
     }
 
     /* ---- phase (i) of cycle t: wavefront w updates its own quarter ---------- */
-    if (flags & 2) {
+    if ((flags & 2) && w < NWAVE) {
         const int32_t D = d.dmax;
         const uint64_t dmask = (D >= 63) ? (~0ull >> 1) : ((1ull << D) - 1ull);
         const int32_t base = k * T + (int32_t)qbase;
@@ -554,6 +571,58 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
             }
         }
     }
+}
+
+/* The row-list kernel, launched after every update: one workgroup per source
+ * chunk (at most BSIM_CHUNK = 4 * ROWS_THREADS pres) builds the chunk's list
+ * of active (pre, delay) rows, ascending, from the history words the update
+ * just wrote: every thread reads the words of 4 consecutive pres (coalesced),
+ * counts their active delays, and a workgroup prefix places the rows.  The
+ * lists are read by every tile that receives from the chunk. */
+__global__ __launch_bounds__(ROWS_THREADS) void bsim_rows_kernel(bsim_dev d)
+{
+    __shared__ uint32_t scr[ROWS_THREADS / WAVE];
+    const int32_t u = blockIdx.x;
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & (WAVE - 1);
+    const uint32_t w = tid / WAVE;
+    const int32_t lo = d.src_lo[u], hi = d.src_hi[u];
+    const int32_t par = (d.clock[0] - 1) & 1;      /* parity of the last update */
+    const uint64_t *hist = d.hist + (int64_t)par * d.n;
+    uint32_t *out = d.src_list + d.src_list_base[u];
+    const int32_t D = d.dmax;
+    const uint64_t dmask = (D >= 63) ? (~0ull >> 1) : ((1ull << D) - 1ull);
+    const int32_t p0 = lo + 4 * (int32_t)tid;
+    uint64_t m[4];
+    uint32_t cnt = 0;
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+        m[x] = (p0 + x < hi) ? ((hist[p0 + x] >> 1) & dmask) : 0ull;
+        cnt += (uint32_t)__popcll(m[x]);
+    }
+    const uint32_t incl = wave_incl_scan(cnt);
+    if (lane == WAVE - 1) scr[w] = incl;
+    __syncthreads();
+    uint32_t pos = incl - cnt, tot = 0;
+#pragma unroll
+    for (int i = 0; i < ROWS_THREADS / WAVE; i++) {
+        const uint32_t x = scr[i];
+        pos += ((uint32_t)i < w) ? x : 0u;
+        tot += x;
+    }
+#pragma unroll
+    for (int x = 0; x < 4; x++) {
+        const uint32_t rb = (uint32_t)(p0 + x - lo) * (uint32_t)D;
+        for (uint64_t mm = m[x]; mm; mm &= mm - 1) out[pos++] = rb + (uint32_t)__builtin_ctzll(mm);
+    }
+    if (tid == 0) d.src_count[u] = tot;
+}
+
+extern "C" hipError_t bsim_launch_rows(const bsim_dev *d, hipStream_t stream)
+{
+    if (d->nsrc <= 0) return hipSuccess;
+    hipLaunchKernelGGL(bsim_rows_kernel, dim3(d->nsrc), dim3(ROWS_THREADS), 0, stream, *d);
+    return hipGetLastError();
 }
 
 /* Orders the tiles by their measured duration, longest first, so that the

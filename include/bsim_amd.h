@@ -140,6 +140,13 @@ int32_t bsim_sim_read_trace(bsim_sim *sim, float *v);
 int32_t bsim_sim_read_state(bsim_sim *sim, float *v, float *ie, float *ii, int32_t *refrac,
                             int32_t *fire_count);
 
+/* Advances n cycles one launch at a time with a pair of HIP events around
+ * each of the cycle's two kernels on the simulation's stream (the step
+ * kernel, then the active-row list kernel), synchronises, and returns their
+ * mean durations in microseconds.  For measurement; bsim_sim_step is the
+ * production path (graph-replayed, no events). */
+int32_t bsim_sim_time_steps(bsim_sim *sim, int32_t n, double *step_us, double *rows_us);
+
 /* Synaptic deliveries performed so far (sum over tiles). */
 int64_t bsim_sim_events(bsim_sim *sim);
 
