@@ -15,7 +15,6 @@
 #include "internal.h"
 #include "network.h"
 
-extern "C" hipError_t bsim_launch_sched(const bsim_dev *d, hipStream_t stream);
 extern "C" hipError_t bsim_launch_rows(const bsim_dev *d, hipStream_t stream);
 extern "C" hipError_t bsim_launch_step(const bsim_dev *d, int32_t t, int32_t flags,
                                        hipStream_t stream);
@@ -337,6 +336,9 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
     UP(src_lo, L.src_lo);
     UP(src_hi, L.src_hi);
     UP(src_list_base, L.src_list_base);
+    UP(chunk_tiles_off, L.chunk_tiles_off);
+    UP(chunk_tiles, L.chunk_tiles);
+    UP(chunk_wrow, L.chunk_wrow);
     d.nsrc = (int32_t)L.src_lo.size();
     /* state */
     UP(v, L.v_init);
@@ -354,16 +356,17 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
         if (!(d.active = dev_alloc<uint64_t>(sim, 2 * nw, e))) goto hip_fail;
         if (!(d.ev_count = dev_alloc<int64_t>(sim, nt, e))) goto hip_fail;
         if (!(sim->ev_fin = dev_alloc<int64_t>(sim, nt, e))) goto hip_fail;
-        if (!(d.clock = dev_alloc<int32_t>(sim, 2, e))) goto hip_fail;
-        hipMemsetAsync(d.clock, 0, 2 * sizeof(int32_t), sim->stream);
+        if (!(d.clock = dev_alloc<int32_t>(sim, 4, e))) goto hip_fail;
+        hipMemsetAsync(d.clock, 0, 4 * sizeof(int32_t), sim->stream);
         {
             std::vector<int32_t> ident(nt);
             for (size_t i = 0; i < nt; i++) ident[i] = (int32_t)i;
             d.perm = dev_upload(sim, ident, e);
             if (e != hipSuccess) goto hip_fail;
         }
-        if (!(d.tload = dev_alloc<uint32_t>(sim, nt, e))) goto hip_fail;
-        hipMemsetAsync(d.tload, 0, nt * sizeof(uint32_t), sim->stream);
+        if (!(d.tile_load = dev_alloc<uint64_t>(sim, nt, e))) goto hip_fail;
+        hipMemsetAsync(d.tile_load, 0, nt * sizeof(uint64_t), sim->stream);
+        if (!(d.tile_cls = dev_alloc<uint8_t>(sim, nt, e))) goto hip_fail;
         const size_t ns = (size_t)(d.nsrc > 0 ? d.nsrc : 1);
         if (!(d.src_count = dev_alloc<uint32_t>(sim, ns, e))) goto hip_fail;
         hipMemsetAsync(d.src_count, 0, ns * sizeof(uint32_t), sim->stream);
@@ -410,9 +413,9 @@ hip_fail:
 }  // extern "C"
 
 /* Cycles per captured graph: runs of this many cycles are replayed from one
- * hipGraph of identical clocked launches (the kernel reads and advances the
- * device-side cycle counter), followed by the tile scheduler, which reorders
- * the tiles by the durations measured in the last cycle. */
+ * hipGraph of identical clocked launches (the step kernel reads and advances
+ * the device-side cycle counter; the row-list kernel after it builds the
+ * next cycle's active-row lists and ranks the tiles). */
 static const int32_t kGraphCycles = 64;
 
 static hipError_t build_graph(bsim_sim *sim)
@@ -424,7 +427,6 @@ static hipError_t build_graph(bsim_sim *sim)
         le = bsim_launch_step(&sim->d, -1, 0, sim->stream);
         if (le == hipSuccess) le = bsim_launch_rows(&sim->d, sim->stream);
     }
-    if (le == hipSuccess) le = bsim_launch_sched(&sim->d, sim->stream);
     hipGraph_t g = nullptr;
     e = hipStreamEndCapture(sim->stream, &g);
     if (le != hipSuccess) e = le;
@@ -466,10 +468,6 @@ int32_t bsim_sim_step(bsim_sim *sim, int32_t n)
         if (e == hipSuccess) e = bsim_launch_rows(&sim->d, sim->stream);
         if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("step launch: ") + hipGetErrorString(e));
         sim->cycle++;
-        if (sim->cycle % kGraphCycles == 0) {
-            e = bsim_launch_sched(&sim->d, sim->stream);
-            if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("sched launch: ") + hipGetErrorString(e));
-        }
     }
     return 0;
 }
@@ -487,10 +485,7 @@ int32_t bsim_sim_time_steps(bsim_sim *sim, int32_t n, double *step_us, double *r
         if (e == hipSuccess) e = hipEventRecord(ev[3 * i + 1], sim->stream);
         if (e == hipSuccess) e = bsim_launch_rows(&sim->d, sim->stream);
         if (e == hipSuccess) e = hipEventRecord(ev[3 * i + 2], sim->stream);
-        if (e == hipSuccess) {
-            sim->cycle++;
-            if (sim->cycle % kGraphCycles == 0) e = bsim_launch_sched(&sim->d, sim->stream);
-        }
+        if (e == hipSuccess) sim->cycle++;
     }
     if (e == hipSuccess) e = hipStreamSynchronize(sim->stream);
     double a = 0.0, b = 0.0;

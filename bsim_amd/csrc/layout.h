@@ -43,7 +43,7 @@ typedef struct bsim_range {
     int32_t lo, hi;     /* pre neurons [lo, hi) of one chunk of a source range of the tile */
     int64_t off_base;   /* index into off_tab of row (lo, 1) */
     int32_t uid;        /* source chunk: index into src_* */
-    int32_t pad;
+    uint32_t wrow;      /* expected deliveries of one active row into the tile, x 16 */
 } bsim_range;
 
 #define BSIM_MAX_RANGE (1 << 26)   /* pres per source range: row index fits 32 bits */
@@ -93,9 +93,14 @@ typedef struct bsim_dev {
     uint32_t *src_count;
     /* accounting / recording */
     int64_t *ev_count;          /* [ntiles] deliveries performed */
-    int32_t *clock;             /* [2]: cycle of the next launch, workgroups finished */
-    const int32_t *perm;        /* [ntiles] tile of each workgroup, heaviest first */
-    uint32_t *tload;            /* [ntiles] last clocked launch: duration of each tile */
+    int32_t *clock;             /* [4]: cycle of the next launch, step workgroups finished,
+                                   row-list workgroups finished */
+    int32_t *perm;              /* [ntiles] tile of each step workgroup, heaviest first */
+    uint64_t *tile_load;        /* [ntiles] expected work of the coming cycle (x 16) */
+    uint8_t *tile_cls;          /* [ntiles] its half-octave class */
+    const int64_t *chunk_tiles_off;  /* [nsrc + 1] tiles that read each chunk: */
+    const int32_t *chunk_tiles;      /*   tile ids */
+    const uint32_t *chunk_wrow;      /*   expected deliveries per active row, x 16 */
     uint64_t *raster;           /* [rec_steps][nwords] fired bitmaps, or NULL */
     int32_t rec_steps;
     int32_t rec_t0;             /* cycle of raster/trace row 0 */

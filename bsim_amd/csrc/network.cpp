@@ -246,7 +246,7 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
             r.hi = plast + 1;
             r.off_base = (int64_t)toff[t].size();
             r.uid = 0;
-            r.pad = 0;
+            r.wrow = 0;
             const int64_t cnt = (int64_t)(r.hi - r.lo) * D * NW + 1;
             const size_t base = toff[t].size();
             toff[t].resize(base + cnt);
@@ -303,7 +303,13 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
                 rc.hi = std::min(r.hi, c + BSIM_CHUNK);
                 rc.off_base = ob + r.off_base + (int64_t)(c - r.lo) * D * NW;
                 rc.uid = u;
-                rc.pad = 0;
+                {
+                    /* synapses of the tile from the chunk, per (pre, delay) row */
+                    const size_t e0 = (size_t)(r.off_base + (int64_t)(c - r.lo) * D * NW);
+                    const size_t e1 = e0 + (size_t)(rc.hi - rc.lo) * D * NW;
+                    const double per_row = (double)(toff[t][e1] - toff[t][e0]) / ((double)(rc.hi - rc.lo) * D);
+                    rc.wrow = (uint32_t)std::min(per_row * 16.0 + 0.5, 4.0e9);
+                }
                 L.rng.push_back(rc);
                 tile_nrng[t]++;
             }
@@ -313,6 +319,22 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
         std::vector<uint32_t>().swap(toff[t]);
     }
     for (int32_t t = 0; t < NT; t++) L.tile_rng[t + 1] = L.tile_rng[t] + tile_nrng[t];
+    /* reverse index: for each chunk, the tiles that read it */
+    const size_t nch = L.src_lo.size();
+    L.chunk_tiles_off.assign(nch + 1, 0);
+    for (const auto &r : L.rng) L.chunk_tiles_off[(size_t)r.uid + 1]++;
+    for (size_t u = 0; u < nch; u++) L.chunk_tiles_off[u + 1] += L.chunk_tiles_off[u];
+    L.chunk_tiles.resize(L.rng.size());
+    L.chunk_wrow.resize(L.rng.size());
+    {
+        std::vector<int64_t> fill(L.chunk_tiles_off.begin(), L.chunk_tiles_off.end() - 1);
+        for (int32_t t = 0; t < NT; t++)
+            for (int32_t ri = L.tile_rng[t]; ri < L.tile_rng[t + 1]; ri++) {
+                const int64_t x = fill[L.rng[ri].uid]++;
+                L.chunk_tiles[x] = t;
+                L.chunk_wrow[x] = L.rng[ri].wrow;
+            }
+    }
     return 0;
 }
 

@@ -60,6 +60,9 @@ struct Layout {
     std::vector<int32_t> src_lo, src_hi;     /* distinct source ranges */
     std::vector<int64_t> src_list_base;      /* list capacity (hi - lo) * D each */
     int64_t src_list_len = 0;
+    std::vector<int64_t> chunk_tiles_off;    /* per chunk: the tiles reading it */
+    std::vector<int32_t> chunk_tiles;
+    std::vector<uint32_t> chunk_wrow;
 };
 
 /* LIFENeuron::reset (src/neuron/lif/LIFENeuron.cpp:123-167) for one neuron,

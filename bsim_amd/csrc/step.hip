@@ -59,9 +59,9 @@
 #define ROWS_THREADS 1024             /* row-list kernel: 4 pres per thread */
 static_assert(BSIM_CHUNK == 4 * ROWS_THREADS, "a source chunk is one row-list workgroup");
 
-/* LDS bytes of a workgroup */
-#define LDS_FIXED (4 * (2 * RCAP * ROWW + 2 * NWAVE + NWAVE * SSTEP * 2 + 2 * NWAVE * WAVE) + \
-                   4 * (MAXR + 1) + 4 + 16 * MAXR)
+/* LDS bytes of a workgroup: lbase, obase | stage | rows | tot | alist | cum | omask */
+#define LDS_FIXED (16 * MAXR + 8 * NWAVE * SSTEP + 4 * (2 * RCAP * ROWW + 2 * NWAVE + NWAVE * SSTEP) + \
+                   4 * (MAXR + 2) + 8 * NWAVE * NGRP * WAVE)
 
 __device__ __forceinline__ uint32_t bern_hash(uint64_t seed, uint32_t id, uint32_t t)
 {
@@ -113,8 +113,8 @@ struct Shared {
     uint32_t *rows;      /* [2][RCAP][ROWW] quarter boundaries of the staged rows, double-buffered */
     uint32_t *tot;       /* [2][NWAVE] stream length of each quarter in the window */
     uint2 *stage;        /* [NWAVE][SSTEP] (local target in quarter, weight bits) */
-    uint32_t *hflag;     /* [NWAVE][WAVE] row-head flags of a 64-position group */
-    uint32_t *hbase;     /* [NWAVE][WAVE] row-head address bases */
+    uint64_t *omask;     /* [NWAVE][NGRP][WAVE] events of a group owned by each lane */
+    uint32_t *alist;     /* [NWAVE][SSTEP] synapse index of each position of a step */
     uint32_t *cum;       /* [MAXR + 1] rows before each source range of the batch */
     int64_t *lbase;      /* [MAXR] first list entry of each source range */
     int64_t *obase;      /* [MAXR] row-table base of each source range */
@@ -149,50 +149,6 @@ __device__ __forceinline__ void load_win(const uint32_t *rows, RowWin &W, int32_
     W.end = pos0 + readlane(incl, WAVE - 1);
 }
 
-/* Maps stream positions [b, b + 64) to synapse indices: lane i gets position
- * b + i.  cur is the address base of the row covering position b - 1.
- * Returns false for positions past the end of the stream. */
-__device__ __forceinline__ bool map_group(const Shared &S, const uint32_t *rows, RowWin &W, uint32_t b,
-                                          uint32_t &cur, uint32_t &addr, int32_t nrows)
-{
-    const uint32_t lane = threadIdx.x & (WAVE - 1);
-    const uint32_t q = threadIdx.x / WAVE;
-    uint32_t *hf = S.hflag + q * WAVE;
-    uint32_t *hb = S.hbase + q * WAVE;
-    const uint32_t p = b + lane;
-    const uint64_t le = (lane == WAVE - 1) ? ~0ull : ((2ull << lane) - 1ull);
-    bool res = false;
-    uint32_t mb = 0;
-    while (true) {
-        /* heads: sub-rows of the window that start inside the group */
-        hf[lane] = 0u;
-        wave_sync();
-        const uint32_t rel = W.pos - b;
-        if (W.len > 0 && rel < WAVE) {
-            hf[rel] = 1u;
-            hb[rel] = W.base;
-        }
-        wave_sync();
-        const uint64_t H = ballot64(hf[lane] != 0u);
-        if (!res && p < W.end) {
-            const uint64_t hq = H & le;
-            if (hq) {
-                mb = hb[63 - __builtin_clzll(hq)];
-            } else {
-                mb = cur;   /* the row covering b - 1 continues */
-            }
-            res = true;
-        }
-        wave_sync();
-        if (W.end >= b + WAVE) break;             /* uniform */
-        if (W.rw + WAVE >= nrows) break;          /* rows exhausted: stream ends */
-        load_win(rows, W, W.rw + WAVE, W.end, nrows);
-    }
-    cur = readlane(mb, WAVE - 1);
-    addr = mb + p;
-    return res;
-}
-
 /* Loaded events of one streaming step: NGRP groups of 64. */
 struct Step {
     uint32_t tl[NGRP];
@@ -200,24 +156,46 @@ struct Step {
     bool v[NGRP];
 };
 
-/* The loads are issued unconditionally (lanes without an event read the
- * tile's first synapse), so that the compiler can count them exactly and the
- * apply of the previous step waits only for its own loads. */
-__device__ __forceinline__ void load_step(const Shared &S, const uint32_t *rows, RowWin &W, uint32_t b0,
-                                          uint32_t tot, uint32_t &cur, int32_t nrows, uint32_t nsyn,
+/* Loads the events of stream positions [b, b + SSTEP): the sub-rows of the
+ * row window that intersect them write the synapse index of each of their
+ * positions into the wavefront's address list (sliding the window over the
+ * staged rows as needed), then lane i of group g loads position
+ * b + 64 g + i.  The loads are issued unconditionally (lanes without an event
+ * read the tile's first synapse), so that the compiler can count them exactly
+ * and the apply of the previous step waits only for its own loads. */
+__device__ __forceinline__ void load_step(const Shared &S, const uint32_t *rows, RowWin &W, uint32_t b,
+                                          uint32_t tot, int32_t nrows, uint32_t nsyn,
                                           const uint16_t *tgt, const float *wts, Step &st)
 {
+    const uint32_t lane = threadIdx.x & (WAVE - 1);
+    const uint32_t q = threadIdx.x / WAVE;
+    uint32_t *al = S.alist + q * SSTEP;
+    const uint32_t e = min(b + SSTEP, tot);
+    if (b < tot) {                                            /* uniform */
+        while (true) {
+            const uint32_t lo = max(W.pos, b), hi = min(W.pos + W.len, e);
+            const uint32_t n = hi > lo ? hi - lo : 0u;
+            const uint32_t a0 = W.base + lo;
+            uint32_t *dst = al + (lo - b);
+            for (uint32_t x = 0; ballot64(x < n) != 0ull; x++)
+                if (x < n) dst[x] = a0 + x;
+            if (W.end >= e) break;                            /* uniform */
+            if (W.rw + WAVE >= nrows) break;
+            load_win(rows, W, W.rw + WAVE, W.end, nrows);
+        }
+    }
+    wave_sync();
 #pragma unroll
     for (int g = 0; g < NGRP; g++) {
-        const uint32_t b = b0 + g * WAVE;
-        uint32_t addr = 0;
-        bool v = false;
-        if (b < tot) v = map_group(S, rows, W, b, cur, addr, nrows);   /* uniform branch */
-        addr = (v && addr < nsyn) ? addr : 0u;
+        const uint32_t pr = g * WAVE + lane;
+        const bool v = b + pr < e;
+        uint32_t addr = v ? al[pr] : 0u;
+        addr = (addr < nsyn) ? addr : 0u;
         st.v[g] = v;
         st.tl[g] = tgt[addr];
         st.w[g] = wts[addr];
     }
+    wave_sync();
 }
 
 /* E and I accumulators of the S local targets lane, lane + 64, ... of the
@@ -228,52 +206,47 @@ struct Acc {
 };
 
 /* Applies one streaming step: every lane applies, in stream order, the events
- * whose target it owns. */
+ * whose target it owns.  For each group, the event in lane i sets bit i of its
+ * owner's mask with an LDS atomic OR (order-free); the owner walks its mask
+ * upwards, i.e. in stream order. */
 template <int S_>
 __device__ __forceinline__ void apply_step(const Shared &S, const Step &st, uint32_t qbase, Acc<S_> &acc)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     const uint32_t q = threadIdx.x / WAVE;
     uint2 *stg = S.stage + q * SSTEP;
+    uint64_t *om = S.omask + q * (NGRP * WAVE);
+    const uint64_t mybit = 1ull << lane;
+#pragma unroll
+    for (int g = 0; g < NGRP; g++) {
+        const uint32_t li = st.tl[g] - qbase;     /* local target within the quarter */
+        stg[g * WAVE + lane] = make_uint2(li, __float_as_uint(st.w[g]));
+        if (st.v[g]) atomicOr((unsigned long long *)&om[g * WAVE + (li & (WAVE - 1))], (unsigned long long)mybit);
+    }
+    wave_sync();
     uint64_t M[NGRP];
 #pragma unroll
     for (int g = 0; g < NGRP; g++) {
-        const bool v = st.v[g];
-        const uint32_t li = st.tl[g] - qbase;     /* local target within the quarter */
-        stg[g * WAVE + lane] = make_uint2(li, __float_as_uint(st.w[g]));
-        uint64_t m = ballot64(v);
-#pragma unroll
-        for (int bit = 0; bit < 6; bit++) {
-            const uint64_t bb = ballot64(v && ((li >> bit) & 1u));
-            m &= ((lane >> bit) & 1u) ? bb : ~bb;
-        }
-        M[g] = m;
+        M[g] = om[g * WAVE + lane];
+        om[g * WAVE + lane] = 0ull;
     }
-    wave_sync();
-    uint64_t m0 = M[0], m1 = M[1], m2 = M[2], m3 = M[3];
-    while ((m0 | m1 | m2 | m3) != 0ull) {
-        uint32_t g;
-        uint64_t mm;
-        if (m0) {
-            g = 0; mm = m0; m0 &= m0 - 1;
-        } else if (m1) {
-            g = 1; mm = m1; m1 &= m1 - 1;
-        } else if (m2) {
-            g = 2; mm = m2; m2 &= m2 - 1;
-        } else {
-            g = 3; mm = m3; m3 &= m3 - 1;
-        }
-        const uint2 e = stg[g * WAVE + (uint32_t)__builtin_ctzll(mm)];
-        const float x = __uint_as_float(e.y);
-        const uint32_t slot = e.x >> 6;
-        const bool exc = x >= 0.0f;
 #pragma unroll
-        for (int j = 0; j < S_; j++) {
-            if (slot == (uint32_t)j) {
-                if (exc)
-                    acc.e[j] = acc.e[j] + x;
-                else
-                    acc.i[j] = acc.i[j] + x;
+    for (int g = 0; g < NGRP; g++) {
+        uint64_t m = M[g];
+        while (m) {
+            const uint2 ev = stg[g * WAVE + (uint32_t)__builtin_ctzll(m)];
+            m &= m - 1;
+            const float x = __uint_as_float(ev.y);
+            const uint32_t slot = ev.x >> 6;
+            const bool exc = x >= 0.0f;
+#pragma unroll
+            for (int j = 0; j < S_; j++) {
+                if (slot == (uint32_t)j) {
+                    if (exc)
+                        acc.e[j] = acc.e[j] + x;
+                    else
+                        acc.i[j] = acc.i[j] + x;
+                }
             }
         }
     }
@@ -290,16 +263,15 @@ __device__ void stream_rows(const Shared &S, const uint32_t *rows, int32_t nrows
     if (tot == 0) return;                                     /* uniform */
     RowWin W;
     load_win(rows, W, 0, 0, nrows);
-    uint32_t cur = 0;
     Step A, B;
-    load_step(S, rows, W, 0, tot, cur, nrows, nsyn, tgt, wts, A);
+    load_step(S, rows, W, 0, tot, nrows, nsyn, tgt, wts, A);
     uint32_t b = 0;
     while (true) {
-        load_step(S, rows, W, b + SSTEP, tot, cur, nrows, nsyn, tgt, wts, B);
+        load_step(S, rows, W, b + SSTEP, tot, nrows, nsyn, tgt, wts, B);
         apply_step<S_>(S, A, qbase, acc);
         b += SSTEP;
         if (b >= tot) break;
-        load_step(S, rows, W, b + SSTEP, tot, cur, nrows, nsyn, tgt, wts, A);
+        load_step(S, rows, W, b + SSTEP, tot, nrows, nsyn, tgt, wts, A);
         apply_step<S_>(S, B, qbase, acc);
         b += SSTEP;
         if (b >= tot) break;
@@ -453,7 +425,6 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
     const int32_t T = d.tile;
     const int32_t k = d.perm[blockIdx.x];
     const uint32_t tid = threadIdx.x;
-    const uint64_t t_begin = wall_clock64();
     const uint32_t lane = tid & (WAVE - 1);
     const uint32_t w = tid / WAVE;
     const uint32_t qbase = w * ((uint32_t)T / NWAVE);
@@ -467,9 +438,9 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
         uint32_t *p = reinterpret_cast<uint32_t *>(p8) + NWAVE * SSTEP * 2;
         S.rows = p;                                   p += 2 * RCAP * ROWW;
         S.tot = p;                                    p += 2 * NWAVE;
-        S.hflag = p;                                  p += NWAVE * WAVE;
-        S.hbase = p;                                  p += NWAVE * WAVE;
-        S.cum = p;
+        S.alist = p;                                  p += NWAVE * SSTEP;
+        S.cum = p;                                    p += MAXR + 2;
+        S.omask = reinterpret_cast<uint64_t *>(p);    /* 8-byte aligned: see LDS_FIXED */
     }
 
     Acc<S_> acc;
@@ -477,6 +448,10 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
     for (int j = 0; j < S_; j++) {
         acc.e[j] = 0.0f;
         acc.i[j] = 0.0f;
+    }
+    if (w < NWAVE) {
+#pragma unroll
+        for (int g = 0; g < NGRP; g++) S.omask[(w * NGRP + g) * WAVE + lane] = 0ull;
     }
 
     /* ---- phase (ii) of cycle t-1 ---------------------------------------- */
@@ -562,14 +537,63 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
     if (clocked) {
         __syncthreads();            /* every wavefront has read d.clock[0] */
         if (tid == 0) {
-            const uint64_t dt = wall_clock64() - t_begin;
-            d.tload[k] = dt > 0xFFFFFFFFull ? 0xFFFFFFFFu : (uint32_t)dt;
             const int32_t done = atomicAdd(&d.clock[1], 1);
             if (done == d.ntiles - 1) {
                 d.clock[1] = 0;
                 d.clock[0] = t + 1;
             }
         }
+    }
+}
+
+/* Orders the tiles by this cycle's expected work, heaviest first, so that the
+ * hardware dispatches the heavy tiles before the light ones (results do not
+ * depend on the order).  A tile's work (accumulated by the row-list
+ * workgroups) is the active-row count of each of its source chunks weighted by
+ * the tile's mean deliveries per row of that chunk plus a fixed cost of two
+ * deliveries.  Counting sort over half-octave classes; within a class the
+ * order is arbitrary.  Runs in one workgroup of ROWS_THREADS threads, which
+ * also clears the loads for the next cycle. */
+#define RANK_CLASSES 128
+
+__device__ __forceinline__ uint32_t load_class(uint64_t x)
+{
+    if (x < 2) return 0;
+    const uint32_t e = 63u - (uint32_t)__builtin_clzll(x);        /* floor(log2 x) >= 1 */
+    const uint32_t h = (uint32_t)(x >> (e - 1)) & 1u;              /* next bit */
+    return min(2u * e + h, (uint32_t)RANK_CLASSES - 1u);
+}
+
+#define RANK_LDS_TILES 16384   /* tile classes kept in LDS up to this many tiles */
+
+__device__ void rank_tiles(const bsim_dev &d)
+{
+    __shared__ uint32_t cnt[RANK_CLASSES], fill[RANK_CLASSES];
+    __shared__ uint8_t cls_lds[RANK_LDS_TILES];
+    const uint32_t tid = threadIdx.x;
+    const bool in_lds = d.ntiles <= RANK_LDS_TILES;
+    uint8_t *cls = in_lds ? cls_lds : d.tile_cls;
+    if (tid < RANK_CLASSES) cnt[tid] = 0u;
+    __syncthreads();
+    for (int32_t k = tid; k < d.ntiles; k += ROWS_THREADS) {
+        /* written by other workgroups' atomics in L2; this workgroup never read it */
+        const uint32_t c = load_class(d.tile_load[k]);
+        cls[k] = (uint8_t)c;
+        d.tile_load[k] = 0ull;
+        atomicAdd(&cnt[c], 1u);
+    }
+    __syncthreads();
+    if (tid == 0) {
+        uint32_t s = 0;
+        for (int c = RANK_CLASSES - 1; c >= 0; c--) {   /* heaviest class first */
+            fill[c] = s;
+            s += cnt[c];
+        }
+    }
+    __syncthreads();
+    for (int32_t k = tid; k < d.ntiles; k += ROWS_THREADS) {
+        const uint32_t pos = atomicAdd(&fill[cls[k]], 1u);
+        d.perm[pos] = k;
     }
 }
 
@@ -616,58 +640,31 @@ __global__ __launch_bounds__(ROWS_THREADS) void bsim_rows_kernel(bsim_dev d)
         for (uint64_t mm = m[x]; mm; mm &= mm - 1) out[pos++] = rb + (uint32_t)__builtin_ctzll(mm);
     }
     if (tid == 0) d.src_count[u] = tot;
+    /* expected work of the tiles that read this chunk: rows weighted by the
+     * tile's mean deliveries per row plus a fixed cost of two */
+    for (int64_t x = d.chunk_tiles_off[u] + tid; x < d.chunk_tiles_off[u + 1]; x += ROWS_THREADS)
+        atomicAdd((unsigned long long *)&d.tile_load[d.chunk_tiles[x]],
+                  (unsigned long long)tot * (unsigned long long)(d.chunk_wrow[x] + 32u));
+
+    /* the last row-list workgroup to finish ranks the tiles for the coming
+     * step launch.  No fence: the ranking only orders the tiles, and any
+     * class assignment yields a permutation of all of them, so a load atomic
+     * still in flight can only make the order less good, never the result
+     * different (device-wide fences write back the L2 and cost more than the
+     * ranking gains). */
+    __shared__ uint32_t last;
+    __syncthreads();                /* this workgroup's atomics are issued */
+    if (tid == 0) last = (uint32_t)(atomicAdd(&d.clock[2], 1) == d.nsrc - 1);
+    __syncthreads();
+    if (!last) return;
+    rank_tiles(d);
+    if (tid == 0) d.clock[2] = 0;
 }
 
 extern "C" hipError_t bsim_launch_rows(const bsim_dev *d, hipStream_t stream)
 {
     if (d->nsrc <= 0) return hipSuccess;
     hipLaunchKernelGGL(bsim_rows_kernel, dim3(d->nsrc), dim3(ROWS_THREADS), 0, stream, *d);
-    return hipGetLastError();
-}
-
-/* Orders the tiles by their measured duration, longest first, so that the
- * hardware dispatches the heavy tiles before the light ones (results do not
- * depend on the order).  One workgroup; a counting sort over 64 log-spaced
- * duration classes (half octaves); within a class the order is arbitrary. */
-#define SCHED_THREADS 1024
-#define SCHED_CLASSES 64
-
-__device__ __forceinline__ uint32_t load_class(uint32_t x)
-{
-    if (x < 2) return 0;
-    const uint32_t e = 31u - (uint32_t)__builtin_clz(x);          /* floor(log2 x) >= 1 */
-    const uint32_t h = (x >> (e - 1)) & 1u;                        /* next bit */
-    return min(2u * e + h, (uint32_t)SCHED_CLASSES - 1u);
-}
-
-__global__ __launch_bounds__(SCHED_THREADS) void bsim_sched_kernel(const uint32_t *tload, int32_t *perm,
-                                                                    int32_t ntiles)
-{
-    __shared__ uint32_t cnt[SCHED_CLASSES], fill[SCHED_CLASSES];
-    const uint32_t tid = threadIdx.x;
-    if (tid < SCHED_CLASSES) cnt[tid] = 0u;
-    __syncthreads();
-    for (int32_t k = tid; k < ntiles; k += SCHED_THREADS) atomicAdd(&cnt[load_class(tload[k])], 1u);
-    __syncthreads();
-    if (tid == 0) {
-        uint32_t s = 0;
-        for (int c = SCHED_CLASSES - 1; c >= 0; c--) {   /* longest class first */
-            fill[c] = s;
-            s += cnt[c];
-        }
-    }
-    __syncthreads();
-    for (int32_t k = tid; k < ntiles; k += SCHED_THREADS) {
-        const uint32_t pos = atomicAdd(&fill[load_class(tload[k])], 1u);
-        perm[pos] = k;
-    }
-}
-
-extern "C" hipError_t bsim_launch_sched(const bsim_dev *d, hipStream_t stream)
-{
-    if (d->ntiles <= 0) return hipSuccess;
-    hipLaunchKernelGGL(bsim_sched_kernel, dim3(1), dim3(SCHED_THREADS), 0, stream, d->tload,
-                       const_cast<int32_t *>(d->perm), d->ntiles);
     return hipGetLastError();
 }
 
