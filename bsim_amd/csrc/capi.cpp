@@ -199,7 +199,10 @@ int32_t bsim_net_layout(const bsim_network *net, float dt, int32_t tile, int32_t
     if (ntiles) *ntiles = L.ntiles;
     if (s) *s = L.s;
     if (dmax) *dmax = L.dmax;
-    if (tile_syn) memcpy(tile_syn, L.tile_syn.data(), L.tile_syn.size() * sizeof(int64_t));
+    if (tile_syn) {
+        for (int32_t t = 0; t < L.ntiles; t++) tile_syn[t] = L.tile_base[t];
+        tile_syn[L.ntiles] = L.s;
+    }
     if (syn_tgt) memcpy(syn_tgt, L.syn_tgt.data(), L.syn_tgt.size() * sizeof(uint16_t));
     if (syn_w) memcpy(syn_w, L.syn_w.data(), L.syn_w.size() * sizeof(float));
     return 0;
@@ -327,7 +330,10 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
     UP(arr_end, L.arr_end);
     UP(arr_t, L.arr_t);
     UP(bern_p, L.bern_p);
-    UP(tile_syn, L.tile_syn);
+    UP(tile_base, L.tile_base);
+    UP(tile_nsyn, L.tile_nsyn);
+    UP(tile_res, L.tile_res);
+    d.nslots = L.nslots;
     UP(syn_tgt, L.syn_tgt);
     UP(syn_w, L.syn_w);
     UP(tile_rng, L.tile_rng);
@@ -359,9 +365,14 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
         if (!(d.clock = dev_alloc<int32_t>(sim, 4, e))) goto hip_fail;
         hipMemsetAsync(d.clock, 0, 4 * sizeof(int32_t), sim->stream);
         {
-            std::vector<int32_t> ident(nt);
-            for (size_t i = 0; i < nt; i++) ident[i] = (int32_t)i;
-            d.perm = dev_upload(sim, ident, e);
+            /* until the first ranking: the tiles of each residue in order */
+            std::vector<int32_t> perm((size_t)(L.nslots > 0 ? L.nslots : 1), -1);
+            std::vector<int32_t> rank(8, 0);
+            for (int32_t t = 0; t < L.ntiles; t++) {
+                const int x = L.tile_res[t];
+                perm[(size_t)8 * rank[x]++ + x] = t;
+            }
+            d.perm = dev_upload(sim, perm, e);
             if (e != hipSuccess) goto hip_fail;
         }
         if (!(d.tile_load = dev_alloc<uint64_t>(sim, nt, e))) goto hip_fail;

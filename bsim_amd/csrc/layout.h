@@ -3,22 +3,23 @@
  *
  * Neurons are numbered in creation order (the order BSim's CPU simulator
  * updates them and reports spikes in).  They are cut into tiles of TILE
- * consecutive neurons (TILE a multiple of 256).  A tile is the unit of work
- * of one workgroup of BSIM_NWAVE wavefronts: it receives the synapses whose
- * target lies in the tile and updates the tile's neurons.  Wavefront q owns
- * the targets of quarter q, local targets [q * TILE/4, (q + 1) * TILE/4), and
- * within it lane l owns the local targets congruent to l modulo 64.
+ * consecutive neurons (TILE a multiple of BSIM_TILE_ALIGN = 64 * BSIM_NWAVE).
+ * A tile is the unit of work of one workgroup: it receives the synapses whose
+ * target lies in the tile and updates the tile's neurons.  Streaming
+ * wavefront q (q < BSIM_NWAVE) owns part q of the tile, local targets
+ * [q * TILE/BSIM_NWAVE, (q + 1) * TILE/BSIM_NWAVE), and within it lane l owns
+ * the local targets congruent to l modulo 64.
  *
  * Synapses are stored tile-major: all synapses into tile k, sorted by
- * (pre, delay_steps, quarter of the target, creation index) -- the (pre, delay)
+ * (pre, delay_steps, part of the target, creation index) -- the (pre, delay)
  * CSR of the reference (src/net/BuildNetwork.cpp:96-176) restricted to the
- * tile, each row split into its four quarters -- as a 16-bit local target and
- * an fp32 weight.  Splitting a row by quarter keeps, for every target, the
+ * tile, each row split into its BSIM_NWAVE parts -- as a 16-bit local target
+ * and an fp32 weight.  Splitting a row by part keeps, for every target, the
  * (pre, delay, creation) order of its inputs.  The pres with synapses into a
  * tile form a few contiguous "source ranges" [lo, hi); each has a table of
- * (hi - lo) * D * 4 + 1 uint32 starts relative to the tile's first synapse:
- * sub-row (p, d, q) is [off[e], off[e + 1]) with
- * e = ((p - lo) * D + d - 1) * 4 + q.
+ * (hi - lo) * D * BSIM_NWAVE + 1 uint32 starts relative to the tile's first
+ * synapse: sub-row (p, d, q) is [off[e], off[e + 1]) with
+ * e = ((p - lo) * D + d - 1) * BSIM_NWAVE + q.
  *
  * Spike history: hist[n] bit j = neuron n fired j cycles ago (bit 0 = this
  * cycle).  After updating cycle t, a tile writes its neurons' history words
@@ -33,9 +34,9 @@
 #include <stdint.h>
 
 #define BSIM_MAX_DELAY 63   /* history word: bits 1..D of a uint64 */
-#define BSIM_NWAVE 4        /* wavefronts per tile = target quarters */
-#define BSIM_TILE_ALIGN 256 /* BSIM_NWAVE quarters of whole 64-lane wavefronts */
-#define BSIM_MAX_TILE 2048  /* 8 bytes of LDS per target, see bsim_step_lds_bytes */
+#define BSIM_NWAVE 3                        /* streaming wavefronts per tile = target parts */
+#define BSIM_TILE_ALIGN (64 * BSIM_NWAVE)   /* parts of whole 64-lane wavefronts */
+#define BSIM_MAX_TILE (8 * BSIM_TILE_ALIGN) /* at most 8 accumulator slots per lane */
 
 enum { BSIM_KIND_LIF = 0, BSIM_KIND_ARRAY = 1, BSIM_KIND_BERNOULLI = 2 };
 
@@ -44,6 +45,8 @@ typedef struct bsim_range {
     int64_t off_base;   /* index into off_tab of row (lo, 1) */
     int32_t uid;        /* source chunk: index into src_* */
     uint32_t wrow;      /* expected deliveries of one active row into the tile, x 16 */
+    uint32_t stride;    /* off_tab entries per (pre, delay) row of the tile's group */
+    uint32_t pad;
 } bsim_range;
 
 #define BSIM_MAX_RANGE (1 << 26)   /* pres per source range: row index fits 32 bits */
@@ -73,7 +76,11 @@ typedef struct bsim_dev {
     uint64_t *hist;             /* [2][n] by cycle parity */
     uint64_t *active;           /* [2][nwords] by cycle parity */
     /* synapses */
-    const int64_t *tile_syn;    /* [ntiles + 1] first synapse of each tile */
+    int32_t nslots;             /* step workgroups (tiles placed by XCD residue) */
+    int32_t pad3;
+    const int64_t *tile_base;   /* [ntiles] first synapse of the tile's group */
+    const uint32_t *tile_nsyn;  /* [ntiles] synapses of the tile's group */
+    const uint8_t *tile_res;    /* [ntiles] XCD residue of the tile's group */
     const uint16_t *syn_tgt;
     const float *syn_w;
     const int32_t *tile_rng;    /* [ntiles + 1] first source range of each tile */
@@ -95,7 +102,7 @@ typedef struct bsim_dev {
     int64_t *ev_count;          /* [ntiles] deliveries performed */
     int32_t *clock;             /* [4]: cycle of the next launch, step workgroups finished,
                                    row-list workgroups finished */
-    int32_t *perm;              /* [ntiles] tile of each step workgroup, heaviest first */
+    int32_t *perm;              /* [nslots] tile of each step workgroup (-1: none) */
     uint64_t *tile_load;        /* [ntiles] expected work of the coming cycle (x 16) */
     uint8_t *tile_cls;          /* [ntiles] its half-octave class */
     const int64_t *chunk_tiles_off;  /* [nsrc + 1] tiles that read each chunk: */

@@ -20,6 +20,7 @@ static const float kZero = 1e-10f;  // `const real ZERO = 1e-10;` (src/utils/con
 /* Two pres with synapses into a tile share a source range when fewer than
  * this many pres without synapses into it lie between them. */
 static const int32_t kRangeGap = 256;
+static const int32_t kGroupTiles = 32;   /* tiles per target group */
 
 void lif_constants(const float *p, float dt, float out[9], int32_t *refrac_time)
 {
@@ -69,13 +70,13 @@ bool synapse_reset(float w, float delay, float tau, float dt, float *w_out, int3
 
 int32_t default_tile(int32_t n, int32_t cus)
 {
-    /* two workgroups (tiles) per compute unit */
-    if (cus <= 0) cus = 256;
-    int64_t want = ((int64_t)n + (int64_t)cus * 2 - 1) / ((int64_t)cus * 2);
-    int64_t t = ((want + BSIM_TILE_ALIGN - 1) / BSIM_TILE_ALIGN) * BSIM_TILE_ALIGN;
-    if (t < BSIM_TILE_ALIGN) t = BSIM_TILE_ALIGN;
-    if (t > BSIM_MAX_TILE) t = BSIM_MAX_TILE;
-    return (int32_t)t;
+    /* The smallest tile: the most workgroups to balance uneven activity over
+     * the compute units, one accumulator slot per lane.  Larger tiles fetch
+     * fewer row-table lines per active row but leave heavy tiles on few
+     * wavefronts. */
+    (void)n;
+    (void)cus;
+    return BSIM_TILE_ALIGN;
 }
 
 int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::string &err)
@@ -205,13 +206,13 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
     std::vector<std::vector<uint32_t>> toff(NT);
     const int32_t D = dmax;
     const int32_t NW = BSIM_NWAVE;
-    const int32_t qw = tile / NW;   /* targets per quarter */
+    const int32_t qw = tile / NW;   /* targets per part */
     int fail = 0;
 
 #pragma omp parallel for schedule(dynamic, 1)
     for (int32_t t = 0; t < NT; t++) {
         const int64_t lo = L.tile_syn[t], hi = L.tile_syn[t + 1], m = hi - lo;
-        /* row key: pre (31 bits) | delay (6 bits) | quarter (2 bits); ties
+        /* row key: pre (31 bits) | delay (6 bits) | part (2 bits); ties
          * keep the creation rank */
         struct Ent { uint64_t key; uint32_t rank; };
         std::vector<Ent> ent(m);
@@ -247,6 +248,8 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
             r.off_base = (int64_t)toff[t].size();
             r.uid = 0;
             r.wrow = 0;
+            r.stride = 0;
+            r.pad = 0;
             const int64_t cnt = (int64_t)(r.hi - r.lo) * D * NW + 1;
             const size_t base = toff[t].size();
             toff[t].resize(base + cnt);
@@ -271,50 +274,142 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
         err = "internal error: CSR row walk mismatch";
         return -7;
     }
-    size_t nr = 0, noff = 0;
+
+    /* Target groups: runs of at most kGroupTiles consecutive tiles with the
+     * same source ranges.  A group's synapses (the same block of the synapse
+     * arrays as its tiles' in tile-major order) are stored pre-major:
+     * (range, pre, delay, tile, part, creation), and its row table per range
+     * holds the part starts of every tile for each (pre, delay) row
+     * consecutively, so that the tiles of a group read shared cache lines. */
+    std::vector<int32_t> gstart;
     for (int32_t t = 0; t < NT; t++) {
-        nr += trng[t].size();
-        noff += toff[t].size();
+        bool same = t > 0 && t - gstart.back() < kGroupTiles && trng[t].size() == trng[t - 1].size();
+        for (size_t i = 0; same && i < trng[t].size(); i++)
+            same = trng[t][i].lo == trng[t - 1][i].lo && trng[t][i].hi == trng[t - 1][i].hi;
+        if (!same) gstart.push_back(t);
     }
+    const int32_t NG = (int32_t)gstart.size();
+    gstart.push_back(NT);
+    for (int32_t g = 0; g < NG; g++) {
+        if (L.tile_syn[gstart[g + 1]] - L.tile_syn[gstart[g]] >= ((int64_t)1 << 32)) {
+            err = "a tile group receives 2^32 or more synapses; use a smaller tile";
+            return -6;
+        }
+    }
+    /* group row tables: per group and range, ((p - lo) * D + d - 1) * nk * NW
+     * + k * NW + q, then one final entry; offsets relative to the group */
+    std::vector<std::vector<int64_t>> gtab_base(NG);
+    std::vector<int64_t> gtab_off((size_t)NG + 1, 0);
+    for (int32_t g = 0; g < NG; g++) {
+        const int32_t nk = gstart[g + 1] - gstart[g];
+        int64_t n = 0;
+        for (const auto &r : trng[gstart[g]]) {
+            gtab_base[g].push_back(n);
+            n += (int64_t)(r.hi - r.lo) * D * nk * NW + 1;
+        }
+        gtab_off[g + 1] = gtab_off[g] + n;
+    }
+    L.off_tab.assign((size_t)gtab_off[NG], 0u);
+    std::vector<uint16_t> gtgt(S);
+    std::vector<float> gw(S);
+#pragma omp parallel for schedule(dynamic, 1)
+    for (int32_t g = 0; g < NG; g++) {
+        const int32_t k0 = gstart[g], nk = gstart[g + 1] - k0;
+        const int64_t gs = L.tile_syn[k0];
+        uint32_t *tab = L.off_tab.data() + gtab_off[g];
+        uint32_t pos = 0;
+        for (size_t ri = 0; ri < trng[k0].size(); ri++) {
+            const bsim_range &r0 = trng[k0][ri];
+            uint32_t *gt = tab + gtab_base[g][ri];
+            size_t x = 0;
+            for (int32_t p = r0.lo; p < r0.hi; p++)
+                for (int32_t d = 1; d <= D; d++)
+                    for (int32_t kk = 0; kk < nk; kk++) {
+                        const int32_t t = k0 + kk;
+                        const size_t e = (size_t)trng[t][ri].off_base +
+                                         ((size_t)(p - r0.lo) * D + (size_t)(d - 1)) * NW;
+                        for (int32_t q = 0; q < NW; q++) {
+                            gt[x++] = pos;
+                            const uint32_t a = toff[t][e + q], b = toff[t][e + q + 1];
+                            for (uint32_t i = a; i < b; i++) {
+                                gtgt[gs + pos] = L.syn_tgt[L.tile_syn[t] + i];
+                                gw[gs + pos] = L.syn_w[L.tile_syn[t] + i];
+                                pos++;
+                            }
+                        }
+                    }
+            gt[x] = pos;
+        }
+    }
+    L.syn_tgt.swap(gtgt);
+    L.syn_w.swap(gw);
+    std::vector<uint16_t>().swap(gtgt);
+    std::vector<float>().swap(gw);
+    L.ngroups = NG;
+    L.tile_base.resize(NT);
+    L.tile_nsyn.resize(NT);
+    L.tile_res.resize(NT);
+    /* XCD residue of each group (workgroups are dispatched to the 8 XCDs
+     * round-robin by workgroup id): groups are spread over the residues to
+     * balance tile counts, so that a group's tiles share one L2 */
+    {
+        std::vector<int64_t> rcount(8, 0);
+        for (int32_t g = 0; g < NG; g++) {
+            const int32_t nk = gstart[g + 1] - gstart[g];
+            const int x = (int)(std::min_element(rcount.begin(), rcount.end()) - rcount.begin());
+            rcount[x] += nk;
+            for (int32_t t = gstart[g]; t < gstart[g + 1]; t++) L.tile_res[t] = (uint8_t)x;
+        }
+        L.nslots = (int32_t)(8 * *std::max_element(rcount.begin(), rcount.end()));
+    }
+
+    size_t nr = 0;
+    for (int32_t t = 0; t < NT; t++) nr += trng[t].size();
     L.rng.reserve(nr);
-    L.off_tab.reserve(noff);
     /* distinct source ranges, each cut into chunks of BSIM_CHUNK pres; a
      * tile's range becomes one entry per chunk */
     std::map<std::pair<int32_t, int32_t>, int32_t> uid;
     std::vector<int32_t> tile_nrng(NT, 0);
-    for (int32_t t = 0; t < NT; t++) {
-        const int64_t ob = (int64_t)L.off_tab.size();
-        for (const auto &r : trng[t]) {
-            auto it = uid.find({r.lo, r.hi});
-            if (it == uid.end()) {
-                it = uid.emplace(std::make_pair(r.lo, r.hi), (int32_t)L.src_lo.size()).first;
-                for (int32_t c = r.lo; c < r.hi; c += BSIM_CHUNK) {
-                    const int32_t ce = std::min(r.hi, c + BSIM_CHUNK);
-                    L.src_lo.push_back(c);
-                    L.src_hi.push_back(ce);
-                    L.src_list_base.push_back(L.src_list_len);
-                    L.src_list_len += (int64_t)(ce - c) * D;
+    for (int32_t g = 0; g < NG; g++) {
+        const int32_t k0 = gstart[g], nk = gstart[g + 1] - k0;
+        for (int32_t t = k0; t < k0 + nk; t++) {
+            L.tile_base[t] = L.tile_syn[k0];
+            L.tile_nsyn[t] = (uint32_t)(L.tile_syn[k0 + nk] - L.tile_syn[k0]);
+            for (size_t ri = 0; ri < trng[t].size(); ri++) {
+                const bsim_range &r = trng[t][ri];
+                auto it = uid.find({r.lo, r.hi});
+                if (it == uid.end()) {
+                    it = uid.emplace(std::make_pair(r.lo, r.hi), (int32_t)L.src_lo.size()).first;
+                    for (int32_t c = r.lo; c < r.hi; c += BSIM_CHUNK) {
+                        const int32_t ce = std::min(r.hi, c + BSIM_CHUNK);
+                        L.src_lo.push_back(c);
+                        L.src_hi.push_back(ce);
+                        L.src_list_base.push_back(L.src_list_len);
+                        L.src_list_len += (int64_t)(ce - c) * D;
+                    }
                 }
-            }
-            int32_t u = it->second;
-            for (int32_t c = r.lo; c < r.hi; c += BSIM_CHUNK, u++) {
-                bsim_range rc;
-                rc.lo = c;
-                rc.hi = std::min(r.hi, c + BSIM_CHUNK);
-                rc.off_base = ob + r.off_base + (int64_t)(c - r.lo) * D * NW;
-                rc.uid = u;
-                {
+                int32_t u = it->second;
+                for (int32_t c = r.lo; c < r.hi; c += BSIM_CHUNK, u++) {
+                    bsim_range rc;
+                    rc.lo = c;
+                    rc.hi = std::min(r.hi, c + BSIM_CHUNK);
+                    rc.stride = (uint32_t)(nk * NW);
+                    rc.off_base = gtab_off[g] + gtab_base[g][ri] +
+                                  (int64_t)(c - r.lo) * D * rc.stride + (int64_t)(t - k0) * NW;
+                    rc.uid = u;
+                    rc.pad = 0;
                     /* synapses of the tile from the chunk, per (pre, delay) row */
                     const size_t e0 = (size_t)(r.off_base + (int64_t)(c - r.lo) * D * NW);
                     const size_t e1 = e0 + (size_t)(rc.hi - rc.lo) * D * NW;
                     const double per_row = (double)(toff[t][e1] - toff[t][e0]) / ((double)(rc.hi - rc.lo) * D);
                     rc.wrow = (uint32_t)std::min(per_row * 16.0 + 0.5, 4.0e9);
+                    L.rng.push_back(rc);
+                    tile_nrng[t]++;
                 }
-                L.rng.push_back(rc);
-                tile_nrng[t]++;
             }
         }
-        L.off_tab.insert(L.off_tab.end(), toff[t].begin(), toff[t].end());
+    }
+    for (int32_t t = 0; t < NT; t++) {
         std::vector<bsim_range>().swap(trng[t]);
         std::vector<uint32_t>().swap(toff[t]);
     }

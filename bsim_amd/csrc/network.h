@@ -51,7 +51,7 @@ struct Layout {
     std::vector<int32_t> refrac_time, arr_cur0, arr_end, arr_t;
     std::vector<uint32_t> bern_p;
     uint64_t seed = 0;
-    std::vector<int64_t> tile_syn;
+    std::vector<int64_t> tile_syn;           /* [ntiles + 1] tile-major synapse counts (prefix) */
     std::vector<uint16_t> syn_tgt;
     std::vector<float> syn_w;
     std::vector<int32_t> tile_rng;
@@ -60,6 +60,10 @@ struct Layout {
     std::vector<int32_t> src_lo, src_hi;     /* distinct source ranges */
     std::vector<int64_t> src_list_base;      /* list capacity (hi - lo) * D each */
     int64_t src_list_len = 0;
+    int32_t ngroups = 0, nslots = 0;         /* target groups; step workgroups */
+    std::vector<int64_t> tile_base;          /* per tile: first synapse of its group */
+    std::vector<uint32_t> tile_nsyn;         /* per tile: synapses of its group */
+    std::vector<uint8_t> tile_res;           /* per tile: XCD residue of its group */
     std::vector<int64_t> chunk_tiles_off;    /* per chunk: the tiles reading it */
     std::vector<int32_t> chunk_tiles;
     std::vector<uint32_t> chunk_wrow;

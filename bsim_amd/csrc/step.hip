@@ -1,42 +1,46 @@
 /*
  * step.hip -- the simulation step on CDNA4 (gfx950).
  *
- * One launch of bsim_step_kernel advances the network by one cycle.  Each
- * 256-thread workgroup (four wavefronts) owns one tile of TILE consecutive
- * neurons (layout.h) and performs, for that tile:
+ * A cycle is two launches.  bsim_step_kernel: one workgroup of BSIM_NWAVE + 1
+ * wavefronts per tile of TILE consecutive neurons (layout.h) performs
  *
  *   (ii) phase (ii) of cycle t-1: every synapse into the tile whose pre fired
- *        d cycles earlier adds its weight to the target's E or I accumulator
- *        (LDS), in the order the reference's CPU simulator applies them
+ *        d cycles earlier adds its weight to the target's E or I accumulator,
+ *        in the order the reference's CPU simulator applies them
  *        (src/net/Network.cpp:324-327, src/neuron/lif/LIFENeuron.cpp:107-116);
  *   (i)  the LIF_brian update of cycle t (src/neuron/lif/LIFENeuron.cpp:169-215),
  *        which consumes exactly those accumulators, the spike history update
  *        and the ordered compaction of the tile's fired / active neurons into
  *        64-bit words by wavefront ballot.
- * The kernel is instantiated for S = TILE / 256 accumulator slots per lane.
  *
- * Phase (ii) for a tile, in windows of up to RCAP rows:
- *   items   the tile's active pres, ascending: the "active" bitmap words of its
- *           source ranges are scanned by all 256 threads (popcount + block
- *           prefix);
- *   rows    for each item its history word gives the active delays; each
- *           active (pre, delay) row is staged in LDS as its four quarter
- *           boundaries, read from the row table;
- *   stream  wavefront q walks the staged rows in order and streams its own
- *           quarter of each: the concatenated sub-rows form a stream of
- *           positions, mapped to synapse addresses 64 at a time (row heads
- *           found by ballot), 256 positions per step, the loads of step s+1
- *           issued before step s is applied;
+ * bsim_rows_kernel then builds, for every source chunk, the list of its
+ * active (pre, delay) rows for the next phase (ii), shared by all tiles that
+ * receive from the chunk, and ranks the tiles by their expected work.
+ *
+ * Phase (ii) for a tile, in windows of up to RCAP rows of the concatenated
+ * lists of its source chunks:
+ *   stage   the producer wavefront (the last one) reads the list entries and
+ *           the row-table entries of window i+1 -- the BSIM_NWAVE + 1 part
+ *           boundaries of each row -- into LDS while the others stream window
+ *           i; its loads stall no streaming wavefront;
+ *   stream  streaming wavefront q walks the staged rows and streams its own
+ *           part of each: the concatenated sub-rows form a stream of
+ *           positions; per step of 256 positions the sub-rows (64 rows at a
+ *           time in the lanes) write the synapse index of each of their
+ *           positions into an LDS address list, and lane i of group g loads
+ *           position 64 g + i; the loads of step s+1 are in flight while step
+ *           s is applied;
  *   apply   the owner of local target x is lane x mod 64 of wavefront q, which
  *           keeps x's E and I accumulators in registers.  For each 64-event
- *           group the lanes ballot the six bits of the owner; from the ballots
- *           every lane forms the mask of the group's events it owns and applies
- *           them in ascending position -- stream order.  No two lanes touch one
- *           accumulator, so there are no atomics, no conflicts and no barriers
- *           between wavefronts; each wavefront then updates its own quarter.
+ *           group, the event in lane i sets bit i of its owner's mask by an
+ *           LDS atomic OR (order-free); every lane walks its mask upwards --
+ *           stream order -- and adds.  No two lanes touch one accumulator, so
+ *           the sums need no atomics, no conflict rounds and no barriers
+ *           between wavefronts; each wavefront then updates its own part.
  * Every accumulator thus receives its adds one at a time in stream order,
  * (pre, delay, creation) -- the creation order the host verified -- so the
- * sums are the reference's sequential fp32 sums, bit for bit.
+ * sums are the reference's sequential fp32 sums, bit for bit.  The kernel is
+ * instantiated for S = TILE / (64 * BSIM_NWAVE) accumulator slots per lane.
  *
  * Arithmetic: compiled with -ffp-contract=off (no FMA contraction) and
  * without denormal flushing, so V = ((Cm*V + v_tmp) + iE*C_E) + iI*C_I rounds
@@ -50,18 +54,19 @@
 #define WAVE 64
 #define NWAVE BSIM_NWAVE              /* streaming wavefronts = target quarters */
 #define NTHREAD (WAVE * (NWAVE + 1))  /* plus one producer wavefront */
-#define RCAP 512                      /* rows per staged window */
+#define RCAP 384                      /* rows per staged window */
 #define RPL (RCAP / WAVE)             /* rows per producer lane */
 #define ROWW (NWAVE + 1)              /* uint32 per staged row: quarter boundaries */
 #define NGRP 4                        /* 64-position groups per streaming step */
 #define SSTEP (NGRP * WAVE)
 #define MAXR 128                      /* source ranges per batch */
 #define ROWS_THREADS 1024             /* row-list kernel: 4 pres per thread */
+#define HEAVY_WGS 256                 /* workgroups run at raised wave priority */
 static_assert(BSIM_CHUNK == 4 * ROWS_THREADS, "a source chunk is one row-list workgroup");
 
-/* LDS bytes of a workgroup: lbase, obase | stage | rows | tot | alist | cum | omask */
+/* LDS bytes of a workgroup: lbase, obase | stage | rows | tot | alist | cum | stride | omask */
 #define LDS_FIXED (16 * MAXR + 8 * NWAVE * SSTEP + 4 * (2 * RCAP * ROWW + 2 * NWAVE + NWAVE * SSTEP) + \
-                   4 * (MAXR + 2) + 8 * NWAVE * NGRP * WAVE)
+                   4 * (MAXR + 2) + 4 * MAXR + 8 * NWAVE * NGRP * WAVE)
 
 __device__ __forceinline__ uint32_t bern_hash(uint64_t seed, uint32_t id, uint32_t t)
 {
@@ -116,6 +121,7 @@ struct Shared {
     uint64_t *omask;     /* [NWAVE][NGRP][WAVE] events of a group owned by each lane */
     uint32_t *alist;     /* [NWAVE][SSTEP] synapse index of each position of a step */
     uint32_t *cum;       /* [MAXR + 1] rows before each source range of the batch */
+    uint32_t *stride;    /* [MAXR] row-table entries per row of each source range */
     int64_t *lbase;      /* [MAXR] first list entry of each source range */
     int64_t *obase;      /* [MAXR] row-table base of each source range */
 };
@@ -316,7 +322,9 @@ __device__ __forceinline__ void prod_stage(const Shared &S, const bsim_dev &d, i
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     uint32_t *rows = S.rows + buf * (RCAP * ROWW);
-    uint32_t ql[NWAVE] = {0, 0, 0, 0};
+    uint32_t ql[NWAVE];
+#pragma unroll
+    for (int q = 0; q < NWAVE; q++) ql[q] = 0u;
 #pragma unroll
     for (int h = 0; h < 2; h++) {
         uint32_t Tb[RPL / 2][ROWW];
@@ -325,7 +333,10 @@ __device__ __forceinline__ void prod_stage(const Shared &S, const bsim_dev &d, i
             const uint32_t jj = lane + WAVE * (h * (RPL / 2) + x);
             const uint32_t j = (uint32_t)win * RCAP + jj;
             int64_t e0 = 0;
-            if (j < nrows_tot) e0 = S.obase[range_of(S, j, nrb)] + (int64_t)E[h * (RPL / 2) + x] * NWAVE;
+            if (j < nrows_tot) {
+                const int32_t ri = range_of(S, j, nrb);
+                e0 = S.obase[ri] + (int64_t)E[h * (RPL / 2) + x] * S.stride[ri];
+            }
 #pragma unroll
             for (int c = 0; c < ROWW; c++) Tb[x][c] = (j < nrows_tot) ? d.off_tab[e0 + c] : 0u;
         }
@@ -370,6 +381,7 @@ __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, in
                 c[x] = d.src_count[r.uid];
                 S.lbase[i] = d.src_list_base[r.uid];
                 S.obase[i] = r.off_base;
+                S.stride[i] = r.stride;
             }
         }
         uint32_t sum = 0;
@@ -422,8 +434,12 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
         t = d.clock[0];
         flags = (t > 0 ? 1 : 0) | 2;
     }
+    /* the heaviest tiles come first (bsim_rows_kernel ranks them): their
+     * wavefronts get issue priority on their compute unit, so that they are
+     * not slowed down by the light tiles that fill the unit around them */
+    if (blockIdx.x < HEAVY_WGS) __builtin_amdgcn_s_setprio(3);
     const int32_t T = d.tile;
-    const int32_t k = d.perm[blockIdx.x];
+    const int32_t k = d.perm[blockIdx.x];       /* -1: an empty slot */
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & (WAVE - 1);
     const uint32_t w = tid / WAVE;
@@ -440,6 +456,7 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
         S.tot = p;                                    p += 2 * NWAVE;
         S.alist = p;                                  p += NWAVE * SSTEP;
         S.cum = p;                                    p += MAXR + 2;
+        S.stride = p;                                 p += MAXR;
         S.omask = reinterpret_cast<uint64_t *>(p);    /* 8-byte aligned: see LDS_FIXED */
     }
 
@@ -455,9 +472,9 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
     }
 
     /* ---- phase (ii) of cycle t-1 ---------------------------------------- */
-    if (flags & 1) {
-        const int64_t syn0 = d.tile_syn[k];
-        const uint32_t nsyn = (uint32_t)(d.tile_syn[k + 1] - syn0);
+    if ((flags & 1) && k >= 0) {
+        const int64_t syn0 = d.tile_base[k];
+        const uint32_t nsyn = d.tile_nsyn[k];
         const uint16_t *tgt = d.syn_tgt + syn0;
         const float *wts = d.syn_w + syn0;
         int64_t events = 0;
@@ -469,7 +486,7 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
     }
 
     /* ---- phase (i) of cycle t: wavefront w updates its own quarter ---------- */
-    if ((flags & 2) && w < NWAVE) {
+    if ((flags & 2) && w < NWAVE && k >= 0) {
         const int32_t D = d.dmax;
         const uint64_t dmask = (D >= 63) ? (~0ull >> 1) : ((1ull << D) - 1ull);
         const int32_t base = k * T + (int32_t)qbase;
@@ -538,7 +555,7 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
         __syncthreads();            /* every wavefront has read d.clock[0] */
         if (tid == 0) {
             const int32_t done = atomicAdd(&d.clock[1], 1);
-            if (done == d.ntiles - 1) {
+            if (done == d.nslots - 1) {
                 d.clock[1] = 0;
                 d.clock[0] = t + 1;
             }
@@ -565,35 +582,45 @@ __device__ __forceinline__ uint32_t load_class(uint64_t x)
 }
 
 #define RANK_LDS_TILES 16384   /* tile classes kept in LDS up to this many tiles */
+#define NRES 8                  /* XCDs: workgroup w runs on XCD (w + c) mod 8 */
 
 __device__ void rank_tiles(const bsim_dev &d)
 {
-    __shared__ uint32_t cnt[RANK_CLASSES], fill[RANK_CLASSES];
+    __shared__ uint32_t cnt[NRES][RANK_CLASSES], fill[NRES][RANK_CLASSES], rtot[NRES];
     __shared__ uint8_t cls_lds[RANK_LDS_TILES];
     const uint32_t tid = threadIdx.x;
     const bool in_lds = d.ntiles <= RANK_LDS_TILES;
     uint8_t *cls = in_lds ? cls_lds : d.tile_cls;
-    if (tid < RANK_CLASSES) cnt[tid] = 0u;
+    for (uint32_t i = tid; i < NRES * RANK_CLASSES; i += ROWS_THREADS) (&cnt[0][0])[i] = 0u;
     __syncthreads();
     for (int32_t k = tid; k < d.ntiles; k += ROWS_THREADS) {
         /* written by other workgroups' atomics in L2; this workgroup never read it */
         const uint32_t c = load_class(d.tile_load[k]);
         cls[k] = (uint8_t)c;
         d.tile_load[k] = 0ull;
-        atomicAdd(&cnt[c], 1u);
+        atomicAdd(&cnt[d.tile_res[k]][c], 1u);
     }
     __syncthreads();
-    if (tid == 0) {
+    if (tid < NRES) {
         uint32_t s = 0;
         for (int c = RANK_CLASSES - 1; c >= 0; c--) {   /* heaviest class first */
-            fill[c] = s;
-            s += cnt[c];
+            fill[tid][c] = s;
+            s += cnt[tid][c];
         }
+        rtot[tid] = s;
     }
     __syncthreads();
+    /* tile of residue x at rank i goes to workgroup NRES * i + x */
     for (int32_t k = tid; k < d.ntiles; k += ROWS_THREADS) {
-        const uint32_t pos = atomicAdd(&fill[cls[k]], 1u);
-        d.perm[pos] = k;
+        const uint32_t x = d.tile_res[k];
+        const uint32_t pos = atomicAdd(&fill[x][cls[k]], 1u);
+        d.perm[NRES * pos + x] = k;
+    }
+    /* the remaining slots of each residue are empty */
+    const int32_t per = d.nslots / NRES;
+    for (int32_t sidx = tid; sidx < d.nslots; sidx += ROWS_THREADS) {
+        const int32_t x = sidx % NRES, i = sidx / NRES;
+        if (i >= (int32_t)rtot[x] && i < per) d.perm[sidx] = -1;
     }
 }
 
@@ -678,7 +705,7 @@ extern "C" hipError_t bsim_launch_step(const bsim_dev *d, int32_t t, int32_t fla
                                        hipStream_t stream)
 {
     if (d->ntiles <= 0) return hipSuccess;
-    const dim3 grid(d->ntiles), block(NTHREAD);
+    const dim3 grid(d->nslots), block(NTHREAD);
     const size_t lds = bsim_step_lds_bytes(d->tile);
     switch (d->tile / (NWAVE * WAVE)) {
     case 1: hipLaunchKernelGGL(bsim_step_kernel<1>, grid, block, lds, stream, *d, t, flags); break;

@@ -101,7 +101,7 @@ int64_t bsim_net_num_synapses(const bsim_network *net);
 
 typedef struct bsim_sim_config {
     int32_t device;        /* HIP device ordinal */
-    int32_t tile;          /* neurons per tile (multiple of 64); 0 = automatic */
+    int32_t tile;          /* neurons per tile (multiple of 192, at most 1536); 0 = automatic */
     int32_t rec_steps;     /* cycles of raster/trace kept on the device (0 = none) */
     int32_t n_rec;         /* neurons whose V is traced */
     const int32_t *rec_ids;
@@ -207,9 +207,12 @@ int64_t bsim_net_export_synapses(const bsim_network *net, int64_t start, int64_t
 int32_t bsim_net_lif_constants(const bsim_network *net, float dt, float *out9,
                                int32_t *refrac_time);
 
-/* Device layout computed on the host, for inspection: per tile synapse
- * ranges and the (pre, delay)-sorted local targets / weights.  Pass NULL
- * arrays to query sizes through *ntiles / *s. */
+/* Device layout computed on the host, for inspection: the local targets /
+ * weights as stored on the device, and per tile (ntiles + 1 entries) the
+ * first synapse of the tile's group -- a run of consecutive tiles with the
+ * same source ranges whose synapses are stored in (range, pre, delay, tile,
+ * part, creation) order.  Pass NULL arrays to query sizes through *ntiles /
+ * *s. */
 int32_t bsim_net_layout(const bsim_network *net, float dt, int32_t tile, int32_t *ntiles,
                         int64_t *s, int32_t *dmax, int64_t *tile_syn, uint16_t *syn_tgt,
                         float *syn_w);

@@ -52,7 +52,7 @@ def assert_same(g, o, what):
 GOLDEN_F32 = [n for n in golden_io.case_names() if n.endswith("_f32")]
 
 
-@pytest.mark.parametrize("tile", [0, 256])
+@pytest.mark.parametrize("tile", [0, 192])
 @pytest.mark.parametrize("name", GOLDEN_F32)
 def test_golden_cases(name, tile):
     """GPU == reference CPU build, bit for bit."""
@@ -76,7 +76,7 @@ def test_golden_cases(name, tile):
 
 
 @pytest.mark.parametrize("seed", range(6))
-@pytest.mark.parametrize("tile", [256, 512, 0])
+@pytest.mark.parametrize("tile", [192, 384, 576, 0])
 def test_random_networks_vs_oracle(seed, tile):
     spec = netspec.random_net(100 + seed, n_lif=300, n_arr=20, n_bern=20, p_conn=0.05,
                               dmax=1 + 7 * seed, allow_dup=seed % 2 == 0, spike_prob=0.1)
@@ -93,7 +93,7 @@ def test_max_delay_63():
     spec = netspec.standard(npop=3, n=300, fanout=40, dmax=63, w0=4.0, drive_rate_hz=300.0,
                             n_drive=100, drive_w=14.0, seed=5, i_offset=(12.0, 14.5))
     say(f"[gpu] dmax63 n={spec.n} s={spec.s}")
-    r = run_gpu(spec, 500, [150, 450, 750], 256)
+    r = run_gpu(spec, 500, [150, 450, 750], 192)
     o = oracle.run(spec, DT, 500, rec_ids=[150, 450, 750])
     assert r["info"]["dmax"] == 63
     assert_same(r, o, "dmax63")
@@ -193,7 +193,7 @@ def dense_net(n_src=700, n_tgt=300, p_fire=0.9, dmax=3, seed=4):
     return spec
 
 
-@pytest.mark.parametrize("tile", [256, 512])
+@pytest.mark.parametrize("tile", [192, 384, 1536])
 def test_dense_overflow_paths(tile):
     spec = dense_net()
     steps = 60

@@ -43,25 +43,39 @@ def test_constants_match_reference(name):
     np.testing.assert_array_equal(rt[lif], g["refrac_time"][lif])
 
 
-def test_layout_is_pre_delay_quarter_sorted():
+def test_layout_is_pre_major_within_groups():
+    """Synapses of a group of tiles are stored in (pre, delay, tile, part,
+    creation) order: each target's inputs keep their (pre, delay, creation)
+    order and the group's tiles share the cache lines of every row."""
     spec = netspec.random_net(5, n_lif=700, p_conn=0.03, dmax=7, allow_dup=True)
     net = api.Network.from_spec(spec)
-    tile = 256
+    tile = 192
     L = net.layout(DT, tile)
     w_ref, ds = oracle.synapse_reset(spec, DT)
-    for k in range(L["ntiles"]):
-        sel = np.nonzero(spec.post // tile == k)[0]          # creation order
-        quarter = (spec.post[sel] - k * tile) // (tile // 4)
-        order = sel[np.lexsort((sel, quarter, ds[sel], spec.pre[sel]))]
-        a, b = L["tile_syn"][k], L["tile_syn"][k + 1]
-        np.testing.assert_array_equal(L["syn_tgt"][a:b], spec.post[order] - k * tile)
+    ts = L["tile_syn"]
+    k = 0
+    ngroups = 0
+    while k < L["ntiles"]:
+        k1 = k
+        while k1 < L["ntiles"] and ts[k1] == ts[k]:
+            k1 += 1
+        a, b = ts[k], (ts[k1] if k1 < L["ntiles"] else L["s"])
+        sel = np.nonzero((spec.post >= k * tile) & (spec.post < k1 * tile))[0]   # creation order
+        tl = spec.post[sel] // tile
+        part = (spec.post[sel] - tl * tile) // (tile // 3)
+        order = sel[np.lexsort((sel, part, tl, ds[sel], spec.pre[sel]))]
+        assert b - a == len(order)
+        np.testing.assert_array_equal(L["syn_tgt"][a:b], spec.post[order] % tile)
         assert np.array_equal(L["syn_w"][a:b].view(np.uint32), w_ref[order].view(np.uint32))
+        ngroups += 1
+        k = k1
+    assert ngroups >= 1
 
 
 def test_layout_single_tile_matches_reference_csr():
     """test/lif_unit1_test.py:105-106: p_dst = [10..19] x 10 + [20..29] x 10."""
     net = api.Network.from_spec(netspec.lif_unit(1))
-    L = net.layout(DT, 256)
+    L = net.layout(DT, 192)
     assert L["ntiles"] == 1 and L["dmax"] == 1
     np.testing.assert_array_equal(L["syn_tgt"], list(range(10, 20)) * 10 + list(range(20, 30)) * 10)
     np.testing.assert_array_equal(L["syn_w"], np.float32([1e-5] * 100 + [2e-5] * 100))
@@ -79,7 +93,7 @@ def _two_neuron(delay_steps, w=0.5, tau=None, dexact=True):
 def test_rejects_unsupported_delay(steps, msg):
     net = api.Network.from_spec(_two_neuron(steps))
     with pytest.raises(api.BsimError, match=msg):
-        net.layout(DT, 256)
+        net.layout(DT, 192)
 
 
 def test_rejects_nonintegral_delay_without_tau():
@@ -88,13 +102,13 @@ def test_rejects_nonintegral_delay_without_tau():
     spec.lif[:, 10:12] = 0.0   # composite taus zero: the reference asserts
     net = api.Network.from_spec(spec)
     with pytest.raises(api.BsimError, match="not a whole number of steps"):
-        net.layout(DT, 256)
+        net.layout(DT, 192)
 
 
 def test_nonintegral_delay_rescales_weight_like_reference():
     spec = _two_neuron(2, dexact=False)
     net = api.Network.from_spec(spec)
-    L = net.layout(DT, 256)
+    L = net.layout(DT, 192)
     w, ds = oracle.synapse_reset(spec, DT)
     assert L["syn_w"][0].view(np.uint32) == w[0].view(np.uint32)
     assert L["dmax"] == ds[0]
@@ -104,16 +118,16 @@ def test_rejects_creation_order_against_pre_order():
     spec = netspec.random_net(1, shuffle=True, n_lif=60, p_conn=0.3)
     net = api.Network.from_spec(spec)
     with pytest.raises(api.BsimError, match="ascending \\(pre, delay\\) order"):
-        net.layout(DT, 256)
+        net.layout(DT, 192)
 
 
 def test_bad_arguments():
     net = api.Network()
     with pytest.raises(api.BsimError):
         net.add_array(np.array([1, 2], np.int64), np.array([0], np.int32))
-    with pytest.raises(api.BsimError, match="multiple of 256"):
+    with pytest.raises(api.BsimError, match="multiple of 192"):
         api.Network.from_spec(netspec.lif_unit(1)).layout(DT, 100)
     with pytest.raises(api.BsimError, match="outside the network"):
         n2 = api.Network.from_spec(netspec.lif_unit(1))
         n2.connect([0], [999], [0.1], [1e-4])
-        n2.layout(DT, 256)
+        n2.layout(DT, 192)
