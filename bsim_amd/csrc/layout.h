@@ -10,16 +10,26 @@
  * [q * TILE/BSIM_NWAVE, (q + 1) * TILE/BSIM_NWAVE), and within it lane l owns
  * the local targets congruent to l modulo 64.
  *
- * Synapses are stored tile-major: all synapses into tile k, sorted by
- * (pre, delay_steps, part of the target, creation index) -- the (pre, delay)
- * CSR of the reference (src/net/BuildNetwork.cpp:96-176) restricted to the
- * tile, each row split into its BSIM_NWAVE parts -- as a 16-bit local target
- * and an fp32 weight.  Splitting a row by part keeps, for every target, the
- * (pre, delay, creation) order of its inputs.  The pres with synapses into a
- * tile form a few contiguous "source ranges" [lo, hi); each has a table of
- * (hi - lo) * D * BSIM_NWAVE + 1 uint32 starts relative to the tile's first
- * synapse: sub-row (p, d, q) is [off[e], off[e + 1]) with
- * e = ((p - lo) * D + d - 1) * BSIM_NWAVE + q.
+ * Target groups: runs of up to 32 consecutive tiles with the same source
+ * ranges (the pres with synapses into the tile, merged over gaps of at most
+ * 256).  A group's synapses form one contiguous block, stored pre-major:
+ * sorted by (source range, pre, delay_steps, tile, part of the target,
+ * creation index) -- the (pre, delay) CSR of the reference
+ * (src/net/BuildNetwork.cpp:96-176) restricted to the group, each row split
+ * by tile and part -- as a 16-bit local target and an fp32 weight.  Sorting
+ * inside a row by tile and part keeps, for every target, the (pre, delay,
+ * creation) order of its inputs; keeping a row's pieces for all tiles of the
+ * group adjacent lets those tiles share cache lines.  Per group and source
+ * range, the row table holds for each (pre, delay) row and each tile of the
+ * group one aligned 16-byte vector (BSIM_NWAVE part starts, then the end of
+ * the tile's sub-rows), offsets relative to the group's first synapse:
+ * vector ((p - lo) * D + d - 1) * ntiles_in_group + k.  A tile's range
+ * entries (bsim_range, one per source chunk of BSIM_CHUNK pres) point at its
+ * vector of row (lo, 1) and give the vector stride per row.
+ *
+ * Workgroups are dispatched to the 8 XCDs round-robin; every group has an XCD
+ * residue and its tiles run in workgroups of that residue, so that they share
+ * the XCD's L2.
  *
  * Spike history: hist[n] bit j = neuron n fired j cycles ago (bit 0 = this
  * cycle).  After updating cycle t, a tile writes its neurons' history words
@@ -42,10 +52,10 @@ enum { BSIM_KIND_LIF = 0, BSIM_KIND_ARRAY = 1, BSIM_KIND_BERNOULLI = 2 };
 
 typedef struct bsim_range {
     int32_t lo, hi;     /* pre neurons [lo, hi) of one chunk of a source range of the tile */
-    int64_t off_base;   /* index into off_tab of row (lo, 1) */
+    int64_t off_base;   /* off_tab index of the tile's vector of row (lo, 1) */
     int32_t uid;        /* source chunk: index into src_* */
     uint32_t wrow;      /* expected deliveries of one active row into the tile, x 16 */
-    uint32_t stride;    /* off_tab entries per (pre, delay) row of the tile's group */
+    uint32_t stride;    /* off_tab entries per (pre, delay) row of the group */
     uint32_t pad;
 } bsim_range;
 
@@ -85,7 +95,7 @@ typedef struct bsim_dev {
     const float *syn_w;
     const int32_t *tile_rng;    /* [ntiles + 1] first source range of each tile */
     const bsim_range *rng;
-    const uint32_t *off_tab;    /* BSIM_NWAVE sub-row starts per (pre, delay) */
+    const uint32_t *off_tab;    /* group row tables (16-byte aligned vectors) */
     /* source chunks and their active-row lists: the distinct source ranges of
      * all tiles, cut into chunks of at most BSIM_CHUNK pres (a tile's range
      * entries are per chunk).  After the update of cycle t, list u holds the

@@ -296,8 +296,13 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
             return -6;
         }
     }
-    /* group row tables: per group and range, ((p - lo) * D + d - 1) * nk * NW
-     * + k * NW + q, then one final entry; offsets relative to the group */
+    /* group row tables: per group and range, entry
+     * ((p - lo) * D + d - 1) * nk * RW + k * RW + q (RW = NW + 1) holds the
+     * start of part q of tile k's sub-rows of row (p, d), entry q = NW its
+     * end: one aligned 16-byte vector per tile and row.  Offsets relative to
+     * the group's first synapse. */
+    const int32_t RW = NW + 1;
+    static_assert(BSIM_NWAVE + 1 == 4, "a tile's row-table entry is one 16-byte vector");
     std::vector<std::vector<int64_t>> gtab_base(NG);
     std::vector<int64_t> gtab_off((size_t)NG + 1, 0);
     for (int32_t g = 0; g < NG; g++) {
@@ -305,7 +310,7 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
         int64_t n = 0;
         for (const auto &r : trng[gstart[g]]) {
             gtab_base[g].push_back(n);
-            n += (int64_t)(r.hi - r.lo) * D * nk * NW + 1;
+            n += (int64_t)(r.hi - r.lo) * D * nk * RW;
         }
         gtab_off[g + 1] = gtab_off[g] + n;
     }
@@ -337,8 +342,8 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
                                 pos++;
                             }
                         }
+                        gt[x++] = pos;
                     }
-            gt[x] = pos;
         }
     }
     L.syn_tgt.swap(gtgt);
@@ -393,9 +398,9 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
                     bsim_range rc;
                     rc.lo = c;
                     rc.hi = std::min(r.hi, c + BSIM_CHUNK);
-                    rc.stride = (uint32_t)(nk * NW);
+                    rc.stride = (uint32_t)(nk * RW);
                     rc.off_base = gtab_off[g] + gtab_base[g][ri] +
-                                  (int64_t)(c - r.lo) * D * rc.stride + (int64_t)(t - k0) * NW;
+                                  (int64_t)(c - r.lo) * D * rc.stride + (int64_t)(t - k0) * RW;
                     rc.uid = u;
                     rc.pad = 0;
                     /* synapses of the tile from the chunk, per (pre, delay) row */

@@ -56,7 +56,7 @@
 #define NTHREAD (WAVE * (NWAVE + 1))  /* plus one producer wavefront */
 #define RCAP 384                      /* rows per staged window */
 #define RPL (RCAP / WAVE)             /* rows per producer lane */
-#define ROWW (NWAVE + 1)              /* uint32 per staged row: quarter boundaries */
+#define ROWW (NWAVE + 1)              /* uint32 per staged row: part starts and end */
 #define NGRP 4                        /* 64-position groups per streaming step */
 #define SSTEP (NGRP * WAVE)
 #define MAXR 128                      /* source ranges per batch */
@@ -115,7 +115,7 @@ __device__ __forceinline__ uint32_t wave_incl_scan(uint32_t x)
 }
 
 struct Shared {
-    uint32_t *rows;      /* [2][RCAP][ROWW] quarter boundaries of the staged rows, double-buffered */
+    uint32_t *rows;      /* [2][RCAP][ROWW] part boundaries of the staged rows, double-buffered */
     uint32_t *tot;       /* [2][NWAVE] stream length of each quarter in the window */
     uint2 *stage;        /* [NWAVE][SSTEP] (local target in quarter, weight bits) */
     uint64_t *omask;     /* [NWAVE][NGRP][WAVE] events of a group owned by each lane */
@@ -303,52 +303,51 @@ __device__ __forceinline__ void prod_entries(const Shared &S, const bsim_dev &d,
                                              uint32_t nrows_tot, int32_t nrb, uint32_t (&E)[RPL])
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
+    int32_t ri = range_of(S, (uint32_t)win * RCAP + lane, nrb);
 #pragma unroll
     for (int x = 0; x < RPL; x++) {
         const uint32_t j = (uint32_t)win * RCAP + lane + WAVE * x;
         E[x] = 0u;
         if (j < nrows_tot) {
-            const int32_t ri = range_of(S, j, nrb);
+            while (ri + 1 < nrb && S.cum[ri + 1] <= j) ri++;
             E[x] = d.src_list[S.lbase[ri] + (j - S.cum[ri])];
         }
     }
 }
 
-/* Stages window `win` into LDS buffer `buf`: the row-table entries of its
- * rows (from the list entries E), and the stream length of each quarter.
- * Loads and stores go in two halves to bound the registers held. */
+/* Stages window `win` into LDS buffer `buf`: the row-table entry of each of
+ * its rows (from the list entries E) -- the part starts and the end of the
+ * tile's sub-rows, one aligned 16-byte load -- and the stream length of each
+ * part.  The rows of a lane ascend, so their source range is found by walking
+ * forward. */
 __device__ __forceinline__ void prod_stage(const Shared &S, const bsim_dev &d, int32_t buf, int32_t win,
                                            uint32_t nrows_tot, int32_t nrb, const uint32_t (&E)[RPL])
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
-    uint32_t *rows = S.rows + buf * (RCAP * ROWW);
+    uint4 *rows = reinterpret_cast<uint4 *>(S.rows + buf * (RCAP * ROWW));
+    uint4 Tb[RPL];
+    int32_t ri = range_of(S, (uint32_t)win * RCAP + lane, nrb);
+#pragma unroll
+    for (int x = 0; x < RPL; x++) {
+        const uint32_t j = (uint32_t)win * RCAP + lane + WAVE * x;
+        Tb[x] = make_uint4(0u, 0u, 0u, 0u);
+        if (j < nrows_tot) {
+            while (ri + 1 < nrb && S.cum[ri + 1] <= j) ri++;
+            const int64_t e0 = S.obase[ri] + (int64_t)E[x] * S.stride[ri];
+            Tb[x] = *reinterpret_cast<const uint4 *>(d.off_tab + e0);
+        }
+    }
     uint32_t ql[NWAVE];
 #pragma unroll
     for (int q = 0; q < NWAVE; q++) ql[q] = 0u;
 #pragma unroll
-    for (int h = 0; h < 2; h++) {
-        uint32_t Tb[RPL / 2][ROWW];
-#pragma unroll
-        for (int x = 0; x < RPL / 2; x++) {
-            const uint32_t jj = lane + WAVE * (h * (RPL / 2) + x);
-            const uint32_t j = (uint32_t)win * RCAP + jj;
-            int64_t e0 = 0;
-            if (j < nrows_tot) {
-                const int32_t ri = range_of(S, j, nrb);
-                e0 = S.obase[ri] + (int64_t)E[h * (RPL / 2) + x] * S.stride[ri];
-            }
-#pragma unroll
-            for (int c = 0; c < ROWW; c++) Tb[x][c] = (j < nrows_tot) ? d.off_tab[e0 + c] : 0u;
-        }
-#pragma unroll
-        for (int x = 0; x < RPL / 2; x++) {
-            const uint32_t jj = lane + WAVE * (h * (RPL / 2) + x);
-            if ((uint32_t)win * RCAP + jj < nrows_tot) {
-#pragma unroll
-                for (int c = 0; c < ROWW; c++) rows[jj * ROWW + c] = Tb[x][c];
-#pragma unroll
-                for (int q = 0; q < NWAVE; q++) ql[q] += Tb[x][q + 1] - Tb[x][q];
-            }
+    for (int x = 0; x < RPL; x++) {
+        const uint32_t jj = lane + WAVE * x;
+        if ((uint32_t)win * RCAP + jj < nrows_tot) {
+            rows[jj] = Tb[x];
+            ql[0] += Tb[x].y - Tb[x].x;
+            ql[1] += Tb[x].z - Tb[x].y;
+            ql[2] += Tb[x].w - Tb[x].z;
         }
     }
 #pragma unroll
