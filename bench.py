@@ -2,6 +2,7 @@
 
     python bench.py [--gpus 1] [--steps K] [--warmup W] [--workload c1|c2|c0]
                     [--cpu-steps C] [--no-cpu]
+    python bench.py --workload c3 [--c3-nets 512] [--c3-solo 32]   (batched sweep)
     python bench.py --copy-probe              (calibration copy kernel for PMC runs)
     python bench.py --pmc-summary DIR [--kernel REGEX]
 
@@ -160,6 +161,65 @@ def read_traffic(workload, tile):
         return None
 
 
+def bench_sweep(a):
+    """Batched parameter sweep: networks/s of one batched simulation of all
+    sweep networks versus the same networks simulated one at a time (timed on
+    a random sample), each sampled network's raster checked bit for bit
+    between the two."""
+    from bsim_amd import sweep
+    size = (a.c3_min_size, a.c3_max_size)
+    pts = sweep.sample(a.c3_nets, seed=a.c3_seed, size=size)
+    steps = a.c3_steps
+    nsyn_est = sum(p.fan_in * sum(p.sizes[1:]) + p.drive_fan_in * p.sizes[0] for p in pts)
+    print(f"[bench] c3: {len(pts)} networks, {sum(p.n for p in pts)} neurons, ~{nsyn_est:.3g} "
+          f"synapses, {steps} cycles each", file=sys.stderr, flush=True)
+    t0 = time.time()
+    tile, tt = sweep.autotune_tile(pts[:a.c3_tune_nets], steps=100)
+    print(f"[bench] autotuned tile {tile} from {tt} ({time.time() - t0:.1f}s)", file=sys.stderr,
+          flush=True)
+    rng = np.random.default_rng(a.c3_seed + 1)
+    sample = sorted(rng.choice(len(pts), size=min(a.c3_solo, len(pts)), replace=False).tolist())
+    # batched, timed without recording, then again with the raster for the check
+    _, secs_b, info = sweep.run_batch(pts, steps, tile=tile, record=False)
+    print(f"[bench] batched: {secs_b:.3f}s for {len(pts)} networks ({info['ntiles']} tiles, "
+          f"{info['events']} events)", file=sys.stderr, flush=True)
+    rast, _, _ = sweep.run_batch([pts[i] for i in sample], steps, tile=tile, record=True)
+    secs_s, exact = 0.0, 0
+    for j, i in enumerate(sample):
+        _, dt_solo = sweep.run_solo(pts[i], steps, tile=tile, record=False)
+        secs_s += dt_solo
+        solo, _ = sweep.run_solo(pts[i], steps, tile=tile, record=True)
+        exact += int(all(np.array_equal(rast[j][c], solo[c]) for c in range(steps)))
+    nps_b = len(pts) / secs_b
+    nps_s = len(sample) / secs_s
+    print(f"[bench] one at a time: {secs_s:.3f}s for {len(sample)} networks; bit-exact "
+          f"{exact}/{len(sample)}", file=sys.stderr, flush=True)
+    out = {
+        "metric": "sweep networks/s (batched)",
+        "value": nps_b,
+        "unit": "networks/s",
+        "n_gpus": 1,
+        "steps": steps,
+        "warmup": 0,
+        "ms_per_step": secs_b / steps * 1e3,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic (seeded sweep sampler)",
+        "config": {"workload": f"batched sweep: {len(pts)} chains of 2-8 LIF_brian populations, "
+                               f"sizes log-uniform [{size[0]}, {size[1]}], fan-in U[100, 1000], "
+                               f"w0 U[0.3, 1.5], inhibitory U[0.1, 0.4], drive U[5, 50] Hz, "
+                               f"delays 1..U{{6..20}}, dt 0.1 ms, {steps} cycles",
+                   "tile": tile, "autotune": {str(k): v for k, v in tt.items()},
+                   "parallelism": "single GPU"},
+        "one_at_a_time_networks_per_s": nps_s,
+        "speedup": nps_b / nps_s,
+        "bit_exact_networks": f"{exact}/{len(sample)}",
+    }
+    print(json.dumps(out), flush=True)
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
@@ -175,6 +235,13 @@ def main():
     ap.add_argument("--pmc-summary", default=None)
     ap.add_argument("--kernel", default="bsim_step_kernel")
     ap.add_argument("--last", type=int, default=0)
+    ap.add_argument("--c3-nets", type=int, default=512)
+    ap.add_argument("--c3-solo", type=int, default=32)
+    ap.add_argument("--c3-tune-nets", type=int, default=64)
+    ap.add_argument("--c3-steps", type=int, default=2000)
+    ap.add_argument("--c3-min-size", type=int, default=300)
+    ap.add_argument("--c3-max-size", type=int, default=3000)
+    ap.add_argument("--c3-seed", type=int, default=7)
     a = ap.parse_args()
     if a.copy_probe:
         copy_probe()
@@ -186,6 +253,9 @@ def main():
         print(f"bench.py runs on one GPU only (got --gpus {a.gpus}); the multi-GPU spike "
               "exchange is not part of this build.", file=sys.stderr)
         sys.exit(2)
+    if a.workload == "c3":
+        bench_sweep(a)
+        return
 
     t0 = time.time()
     net, desc = make_workload(a.workload)

@@ -36,6 +36,8 @@ _SIGS = {
     "bsim_net_add_lif": (C.c_int32, [P, C.c_int32, P]),
     "bsim_net_add_array": (C.c_int32, [P, C.c_int32, P, P, P]),
     "bsim_net_add_bernoulli": (C.c_int32, [P, C.c_int32, P, C.c_uint64]),
+    "bsim_net_add_bernoulli_keyed": (C.c_int32, [P, C.c_int32, P, C.c_uint64, C.c_uint32]),
+    "bsim_net_gen_chain": (C.c_int32, [P, P]),
     "bsim_net_connect": (C.c_int64, [P, C.c_int64, P, P, P, P, P, P]),
     "bsim_net_num_neurons": (C.c_int32, [P]),
     "bsim_net_num_synapses": (C.c_int64, [P]),
@@ -109,6 +111,15 @@ class GenParams(C.Structure):
                 ("v_thresh", C.c_float), ("dt", C.c_float)]
 
 
+class ChainParams(C.Structure):
+    _fields_ = [("npop", C.c_int32), ("sizes", C.c_int32 * 8), ("fan_in", C.c_int32),
+                ("w0", C.c_float), ("inh_frac", C.c_float), ("dmax", C.c_int32),
+                ("drive_rate_hz", C.c_float), ("n_drive", C.c_int32), ("drive_fan_in", C.c_int32),
+                ("drive_w", C.c_float), ("seed", C.c_uint64), ("tau_refrac", C.c_float),
+                ("ioff_lo", C.c_float), ("ioff_hi", C.c_float), ("tau_syn_E", C.c_float),
+                ("tau_syn_I", C.c_float), ("v_thresh", C.c_float), ("dt", C.c_float)]
+
+
 class Network:
     """Host network (bsim_network).  Neuron ids are creation order."""
 
@@ -130,6 +141,24 @@ class Network:
     @property
     def s(self):
         return lib().bsim_net_num_synapses(self._h)
+
+    def add_chain(self, sizes, fan_in, w0, inh_frac, dmax, drive_rate_hz, n_drive, drive_fan_in,
+                  drive_w, seed, tau_refrac=2e-3, i_offset=(14.5, 16.5), tau_syn_E=5e-3,
+                  tau_syn_I=10e-3, v_thresh=15.0, dt=netspec.DT):
+        """Appends one sweep network (bsim_net_gen_chain); returns its first id."""
+        cp = ChainParams()
+        cp.npop = len(sizes)
+        for i, x in enumerate(sizes):
+            cp.sizes[i] = int(x)
+        cp.fan_in, cp.w0, cp.inh_frac, cp.dmax = int(fan_in), w0, inh_frac, int(dmax)
+        cp.drive_rate_hz, cp.n_drive, cp.drive_fan_in = drive_rate_hz, int(n_drive), int(drive_fan_in)
+        cp.drive_w, cp.seed, cp.tau_refrac = drive_w, int(seed), tau_refrac
+        cp.ioff_lo, cp.ioff_hi = i_offset
+        cp.tau_syn_E, cp.tau_syn_I, cp.v_thresh, cp.dt = tau_syn_E, tau_syn_I, v_thresh, dt
+        r = lib().bsim_net_gen_chain(self._h, C.byref(cp))
+        if r < 0:
+            _err(r, "bsim_net_gen_chain")
+        return r
 
     def add_lif(self, params):
         p = np.ascontiguousarray(np.atleast_2d(params), np.float32)

@@ -73,6 +73,8 @@ int32_t bsim_net_add_lif(bsim_network *net, int32_t n, const float *params)
         h.lif.insert(h.lif.end(), params, params + (size_t)n * bsim::kLifParams);
         h.arr_off.insert(h.arr_off.end(), (size_t)n, h.arr_off.back());
         h.bern_p.insert(h.bern_p.end(), (size_t)n, 0u);
+        h.bern_seed.insert(h.bern_seed.end(), (size_t)n, 0ull);
+        h.bern_key.insert(h.bern_key.end(), (size_t)n, 0u);
     } catch (...) {
         return fail(BSIM_E_NOMEM, "out of host memory");
     }
@@ -108,6 +110,8 @@ int32_t bsim_net_add_array(bsim_network *net, int32_t n, const int64_t *offsets,
             h.arr_off.push_back(base + offsets[i + 1] - offsets[i]);
         }
         h.bern_p.insert(h.bern_p.end(), (size_t)n, 0u);
+        h.bern_seed.insert(h.bern_seed.end(), (size_t)n, 0ull);
+        h.bern_key.insert(h.bern_key.end(), (size_t)n, 0u);
     } catch (...) {
         return fail(BSIM_E_NOMEM, "out of host memory");
     }
@@ -115,6 +119,13 @@ int32_t bsim_net_add_array(bsim_network *net, int32_t n, const int64_t *offsets,
 }
 
 int32_t bsim_net_add_bernoulli(bsim_network *net, int32_t n, const uint32_t *p, uint64_t seed)
+{
+    if (!net) return fail(BSIM_E_ARG, "bsim_net_add_bernoulli: bad argument");
+    return bsim_net_add_bernoulli_keyed(net, n, p, seed, (uint32_t)net->net.n());
+}
+
+int32_t bsim_net_add_bernoulli_keyed(bsim_network *net, int32_t n, const uint32_t *p, uint64_t seed,
+                                     uint32_t first_key)
 {
     if (!net || n < 0 || (n > 0 && !p)) return fail(BSIM_E_ARG, "bsim_net_add_bernoulli: bad argument");
     auto &h = net->net;
@@ -129,6 +140,8 @@ int32_t bsim_net_add_bernoulli(bsim_network *net, int32_t n, const uint32_t *p, 
         }
         h.arr_off.insert(h.arr_off.end(), (size_t)n, h.arr_off.back());
         h.bern_p.insert(h.bern_p.end(), p, p + n);
+        h.bern_seed.insert(h.bern_seed.end(), (size_t)n, seed);
+        for (int32_t i = 0; i < n; i++) h.bern_key.push_back(first_key + (uint32_t)i);
         h.seed = seed;
     } catch (...) {
         return fail(BSIM_E_NOMEM, "out of host memory");
@@ -314,7 +327,6 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
     d.ntiles = L.ntiles;
     d.dmax = L.dmax;
     d.nwords = (L.n + 63) / 64;
-    d.seed = L.seed;
     sim->s = L.s;
     sim->n_src = (int64_t)L.rng.size();
     sim->n_off = (int64_t)L.off_tab.size();
@@ -330,6 +342,8 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
     UP(arr_end, L.arr_end);
     UP(arr_t, L.arr_t);
     UP(bern_p, L.bern_p);
+    UP(bern_seed, L.bern_seed);
+    UP(bern_key, L.bern_key);
     UP(tile_base, L.tile_base);
     UP(tile_nsyn, L.tile_nsyn);
     UP(tile_res, L.tile_res);

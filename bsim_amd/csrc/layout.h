@@ -77,7 +77,8 @@ typedef struct bsim_dev {
     const int32_t *arr_end;     /* array sources: end index into arr_t */
     const int32_t *arr_t;
     const uint32_t *bern_p;
-    uint64_t seed;
+    const uint64_t *bern_seed;  /* Bernoulli draw of neuron i at cycle t: */
+    const uint32_t *bern_key;   /*   hash(bern_seed[i], bern_key[i], t) < bern_p[i] */
     /* neuron state */
     float *v, *ie, *ii;
     int32_t *refrac;

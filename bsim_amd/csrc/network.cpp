@@ -97,7 +97,6 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
     L.s = S;
     L.tile = tile;
     L.ntiles = N > 0 ? (N + tile - 1) / tile : 0;
-    L.seed = net.seed;
     L.kind = net.kind;
     L.Cm.assign(N, 0.f); L.CE.assign(N, 0.f); L.CI.assign(N, 0.f);
     L.C_E.assign(N, 0.f); L.C_I.assign(N, 0.f); L.v_tmp.assign(N, 0.f);
@@ -106,6 +105,8 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
     L.arr_cur0.assign(N, 0);
     L.arr_end.assign(N, 0);
     L.bern_p.assign(N, 0u);
+    L.bern_seed.assign(N, 0ull);
+    L.bern_key.assign(N, 0u);
     for (int32_t i = 0; i < N; i++) {
         if (net.kind[i] == BSIM_KIND_LIF) {
             float c[9];
@@ -117,6 +118,8 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
             L.arr_end[i] = (int32_t)net.arr_off[i + 1];
         } else if (net.kind[i] == BSIM_KIND_BERNOULLI) {
             L.bern_p[i] = net.bern_p[i];
+            L.bern_seed[i] = net.bern_seed[i];
+            L.bern_key[i] = net.bern_key[i];
         } else {
             err = "unknown neuron kind";
             return -1;

@@ -33,7 +33,9 @@ struct HostNet {
     std::vector<int64_t> arr_off{0};   /* n + 1, into arr_t */
     std::vector<int32_t> arr_t;
     std::vector<uint32_t> bern_p;
-    uint64_t seed = 0;
+    std::vector<uint64_t> bern_seed;   /* per neuron: seed and key of its Bernoulli */
+    std::vector<uint32_t> bern_key;    /* draws, hash(seed, key, t) (0 for others) */
+    uint64_t seed = 0;                 /* seed of the last add_bernoulli (export) */
     std::vector<int32_t> pre, post;
     std::vector<float> w, d;
     std::vector<int8_t> stype;
@@ -50,7 +52,8 @@ struct Layout {
     std::vector<float> Cm, CE, CI, C_E, C_I, v_tmp, v_thresh, v_reset, v_init;
     std::vector<int32_t> refrac_time, arr_cur0, arr_end, arr_t;
     std::vector<uint32_t> bern_p;
-    uint64_t seed = 0;
+    std::vector<uint64_t> bern_seed;
+    std::vector<uint32_t> bern_key;
     std::vector<int64_t> tile_syn;           /* [ntiles + 1] tile-major synapse counts (prefix) */
     std::vector<uint16_t> syn_tgt;
     std::vector<float> syn_w;

@@ -532,7 +532,7 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
                         d.arr_cur[nid] = c + 1;
                     }
                 } else {
-                    fired = bern_hash(d.seed, (uint32_t)nid, (uint32_t)t) < d.bern_p[nid];
+                    fired = bern_hash(d.bern_seed[nid], d.bern_key[nid], (uint32_t)t) < d.bern_p[nid];
                 }
                 h = (hist_in[nid] << 1) | (fired ? 1ull : 0ull);
                 hist_out[nid] = h;

@@ -1,0 +1,171 @@
+"""Batched parameter sweeps: many independent networks advanced together.
+
+A sweep point is one feed-forward chain network (bsim_net_gen_chain).  A
+batch appends all of its networks to one host network -- disjoint neuron id
+ranges, no synapse between networks, every draw keyed by network-local
+indices -- and simulates them with one step launch per cycle (graph-replayed
+by bsim_sim_step).  Each network's raster is the slice of the batch raster at
+its id range, shifted to local ids, and is bit for bit the raster of the same
+network simulated alone: a target's inputs, their order and the Bernoulli
+draws do not depend on the other networks.
+
+Sweep ranges follow the batched-sweep workload: 2-8 populations of
+log-uniform size, uniform fan-in, w0, inhibitory fraction, Poisson drive rate
+and maximum delay; see ``C3``.
+"""
+from __future__ import annotations
+
+import time
+from dataclasses import dataclass, field
+
+import numpy as np
+
+from . import api, netspec
+
+# parameter ranges of the batched-sweep workload (512 networks, 0.2 s each)
+C3 = dict(n_nets=512, npop=(2, 8), size=(1000, 50000), fan_in=(100, 1000), w0=(0.3, 1.5),
+          inh_frac=(0.1, 0.4), drive_hz=(5.0, 50.0), dmax=(6, 20), steps=2000)
+
+
+@dataclass
+class Point:
+    """One sweep point (the arguments of Network.add_chain)."""
+    sizes: list
+    fan_in: int
+    w0: float
+    inh_frac: float
+    dmax: int
+    drive_rate_hz: float
+    n_drive: int
+    drive_fan_in: int
+    drive_w: float
+    seed: int
+
+    def kwargs(self):
+        return dict(sizes=self.sizes, fan_in=self.fan_in, w0=self.w0, inh_frac=self.inh_frac,
+                    dmax=self.dmax, drive_rate_hz=self.drive_rate_hz, n_drive=self.n_drive,
+                    drive_fan_in=self.drive_fan_in, drive_w=self.drive_w, seed=self.seed)
+
+    @property
+    def n(self):
+        return self.n_drive + int(sum(self.sizes))
+
+    def features(self):
+        """The 6 swept parameters (for the surrogate): w0, inh_frac, drive
+        rate, dmax, fan-in, mean population size."""
+        return np.array([self.w0, self.inh_frac, self.drive_rate_hz, self.dmax, self.fan_in,
+                         float(np.mean(self.sizes))], np.float64)
+
+
+def sample(n_nets, seed=0, npop=C3["npop"], size=C3["size"], fan_in=C3["fan_in"], w0=C3["w0"],
+           inh_frac=C3["inh_frac"], drive_hz=C3["drive_hz"], dmax=C3["dmax"], n_drive=None,
+           drive_fan_in=50, drive_w=2.0):
+    """n_nets sweep points drawn from the given ranges (seeded)."""
+    rng = np.random.default_rng(seed)
+    pts = []
+    for i in range(n_nets):
+        k = int(rng.integers(npop[0], npop[1] + 1))
+        sizes = np.exp(rng.uniform(np.log(size[0]), np.log(size[1]), k)).astype(np.int64)
+        sizes = [int(max(1, x)) for x in sizes]
+        nd = sizes[0] if n_drive is None else int(n_drive)
+        pts.append(Point(sizes=sizes, fan_in=int(rng.integers(fan_in[0], fan_in[1] + 1)),
+                         w0=float(np.float32(rng.uniform(*w0))),
+                         inh_frac=float(np.float32(rng.uniform(*inh_frac))),
+                         dmax=int(rng.integers(dmax[0], dmax[1] + 1)),
+                         drive_rate_hz=float(np.float32(rng.uniform(*drive_hz))),
+                         n_drive=nd, drive_fan_in=int(drive_fan_in), drive_w=float(drive_w),
+                         seed=int(rng.integers(1, 2**62))))
+    return pts
+
+
+@dataclass
+class Batch:
+    """A host network holding several sweep networks, and where each starts."""
+    net: api.Network
+    points: list
+    first: np.ndarray = field(default=None)
+
+    @property
+    def sizes(self):
+        return np.array([p.n for p in self.points], np.int64)
+
+
+def build(points):
+    """All points appended to one host network."""
+    net = api.Network()
+    first = np.array([net.add_chain(**p.kwargs()) for p in points], np.int64)
+    return Batch(net=net, points=list(points), first=first)
+
+
+def split_raster(counts, ids, first, sizes):
+    """Per network: list over cycles of ascending local fired ids."""
+    off = np.concatenate([[0], np.cumsum(counts)])
+    out = [[] for _ in first]
+    order = np.argsort(first)
+    starts = first[order]
+    for c in range(len(counts)):
+        cyc = ids[off[c]:off[c + 1]]
+        b = np.searchsorted(starts, cyc, side="right") - 1
+        for j in np.unique(b):
+            k = order[j]
+            sel = cyc[b == j]
+            out[k].append((c, sel - first[k]))
+    res = []
+    for k in range(len(first)):
+        per = [np.zeros(0, np.int32) for _ in range(len(counts))]
+        for c, sel in out[k]:
+            per[c] = sel.astype(np.int32)
+        res.append(per)
+    return res
+
+
+def run_batch(points, steps, tile=0, record=True):
+    """Simulates the points together; returns (per-network rasters or None,
+    seconds of the stepping, simulation info)."""
+    b = build(points)
+    sim = api.Simulation(b.net, netspec.DT, tile=tile, rec_steps=steps if record else 0)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.finish()
+    sim.sync()
+    secs = time.perf_counter() - t0
+    info = sim.info()
+    rasters = None
+    if record:
+        counts, ids, _ = sim.raster()
+        rasters = split_raster(counts, ids, b.first, b.sizes)
+    ev = sim.events()
+    sim.close()
+    return rasters, secs, dict(info, events=ev)
+
+
+def run_solo(point, steps, tile=0, record=True):
+    """Simulates one point alone; returns (raster or None, seconds of stepping)."""
+    net = api.Network()
+    net.add_chain(**point.kwargs())
+    sim = api.Simulation(net, netspec.DT, tile=tile, rec_steps=steps if record else 0)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.finish()
+    sim.sync()
+    secs = time.perf_counter() - t0
+    raster = sim.raster()[2] if record else None
+    sim.close()
+    return raster, secs
+
+
+def autotune_tile(points, steps=100, tiles=(192, 384, 768)):
+    """The project's tile autotuner: the tile size with the fastest short run
+    of the batch (no recording).  Returns (best tile, {tile: seconds})."""
+    times = {}
+    b = build(points)
+    for t in tiles:
+        sim = api.Simulation(b.net, netspec.DT, tile=t)
+        sim.step(min(64, steps))          # warm-up: first launches, graph capture
+        sim.sync()
+        t0 = time.perf_counter()
+        sim.step(steps)
+        sim.sync()
+        times[t] = time.perf_counter() - t0
+        sim.close()
+    return min(times, key=times.get), times

@@ -84,8 +84,15 @@ int32_t bsim_net_add_array(bsim_network *net, int32_t n, const int64_t *offsets,
                            const int32_t *cycles, const float *taus);
 
 /* Appends n seeded Bernoulli sources firing at cycle t iff
- * hash(seed, id, t) < p[i]; the seed is shared by the network. */
+ * hash(seed, id, t) < p[i], id = the neuron's global id. */
 int32_t bsim_net_add_bernoulli(bsim_network *net, int32_t n, const uint32_t *p, uint64_t seed);
+
+/* The same with explicit keys: source i fires at cycle t iff
+ * hash(seed, first_key + i, t) < p[i].  A network appended to a batch with
+ * its own seed and keys 0.. draws exactly what it draws alone.  (No
+ * counterpart in the reference, whose Poisson sources are clock-seeded.) */
+int32_t bsim_net_add_bernoulli_keyed(bsim_network *net, int32_t n, const uint32_t *p, uint64_t seed,
+                                     uint32_t first_key);
 
 /* Appends count synapses in the given order (= creation order).  d is the
  * delay in seconds; stype (NULL: all excitatory) and tau (NULL: all 0) as in
@@ -192,6 +199,33 @@ typedef struct bsim_gen_params {
  * builds the same topology from all-to-all projections
  * (test/gpu/standard_test.cpp:92-139).  Returns the number of synapses. */
 int64_t bsim_net_gen_standard(bsim_network *net, const bsim_gen_params *p);
+
+/* One network of a parameter sweep: a feed-forward chain of npop LIF_brian
+ * populations (tau_m 20 ms, R 1, v_rest = v_reset = 0, i_offset ~ U[ioff_lo,
+ * ioff_hi)); population i projects to i+1 with about fan_in synapses per
+ * target (per pre round(fan_in * n_dst / n_src) distinct targets by
+ * stratified sampling); exc. w ~ U[0, 0.7 w0), inh. (share inh_frac)
+ * w ~ U[-0.5 w0, 0); delays U{1..dmax} steps; n_drive Bernoulli sources at
+ * drive_rate_hz feed population 0 with about drive_fan_in synapses per target
+ * of weight drive_w. */
+typedef struct bsim_chain_params {
+    int32_t npop;                 /* 1..8 */
+    int32_t sizes[8];
+    int32_t fan_in;
+    float w0, inh_frac;
+    int32_t dmax;
+    float drive_rate_hz;
+    int32_t n_drive, drive_fan_in;
+    float drive_w;
+    uint64_t seed;
+    float tau_refrac, ioff_lo, ioff_hi, tau_syn_E, tau_syn_I, v_thresh;
+    float dt;
+} bsim_chain_params;
+
+/* Appends one sweep network to net (which may hold others: a batch) and
+ * returns the id of its first neuron.  Its draws, Bernoulli keys included,
+ * depend only on p, so the network behaves in a batch exactly as alone. */
+int32_t bsim_net_gen_chain(bsim_network *net, const bsim_chain_params *p);
 
 /* Copies out the network as stored (creation order); NULL arrays skipped.
  * lif receives n * 12 floats. */

@@ -1,0 +1,93 @@
+"""Batched sweeps (bsim_amd/sweep.py): many networks in one simulation, each
+bit-exact against itself simulated alone, and against the oracle."""
+import numpy as np
+import pytest
+
+import oracle
+from bsim_amd import api, netspec, sweep
+
+DT = netspec.DT
+
+SMALL = dict(npop=(2, 4), size=(150, 1500), fan_in=(20, 120), dmax=(6, 20), drive_fan_in=20)
+
+
+def test_chain_is_the_same_alone_and_in_a_batch():
+    """A chain appended after other chains is the same network, neuron for
+    neuron and synapse for synapse (ids shifted), as the chain built alone."""
+    pts = sweep.sample(4, seed=3, **SMALL)
+    b = sweep.build(pts)
+    full = b.net.export_spec()
+    for k, p in enumerate(pts):
+        alone = api.Network()
+        assert alone.add_chain(**p.kwargs()) == 0
+        a = alone.export_spec()
+        f0, n = int(b.first[k]), p.n
+        assert np.array_equal(full.kind[f0:f0 + n], a.kind)
+        assert np.array_equal(full.lif[f0:f0 + n], a.lif)
+        assert np.array_equal(full.bern_p[f0:f0 + n], a.bern_p)
+        sel = (full.pre >= f0) & (full.pre < f0 + n)
+        assert np.array_equal(full.pre[sel] - f0, a.pre)
+        assert np.array_equal(full.post[sel] - f0, a.post)
+        assert np.array_equal(full.w[sel].view(np.uint32), a.w.view(np.uint32))
+        assert np.array_equal(full.d[sel].view(np.uint32), a.d.view(np.uint32))
+        assert np.array_equal(full.stype[sel], a.stype)
+
+
+def test_chain_respects_fan_in_and_pre_order():
+    p = sweep.sample(1, seed=5, **SMALL)[0]
+    net = api.Network()
+    net.add_chain(**p.kwargs())
+    s = net.export_spec()
+    _, ds = oracle.synapse_reset(s, DT)
+    assert oracle.csr_order_ok(s, ds)
+    start = p.n_drive
+    for i in range(1, len(p.sizes)):
+        lo, hi = start + sum(p.sizes[:i]), start + sum(p.sizes[:i + 1])
+        fin = np.bincount(s.post[(s.post >= lo) & (s.post < hi)] - lo, minlength=hi - lo)
+        assert abs(fin.mean() - p.fan_in) <= 0.5 * p.fan_in + 1
+
+
+def test_split_raster():
+    counts = np.array([3, 0, 2], np.int32)
+    ids = np.array([1, 5, 12, 4, 11], np.int32)
+    r = sweep.split_raster(counts, ids, np.array([0, 4, 10]), np.array([4, 6, 5]))
+    assert [list(x) for x in r[0]] == [[1], [], []]
+    assert [list(x) for x in r[1]] == [[1], [], [0]]
+    assert [list(x) for x in r[2]] == [[2], [], [1]]
+
+
+def test_sample_is_seeded():
+    a = sweep.sample(5, seed=9)
+    b = sweep.sample(5, seed=9)
+    assert [p.kwargs() for p in a] == [p.kwargs() for p in b]
+    for p in a:
+        assert 2 <= len(p.sizes) <= 8 and 100 <= p.fan_in <= 1000 and 6 <= p.dmax <= 20
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile", [0, 384])
+def test_batch_rasters_equal_solo_rasters(tile):
+    steps = 300
+    pts = sweep.sample(10, seed=11, **SMALL)
+    rasters, _, info = sweep.run_batch(pts, steps, tile=tile)
+    total = 0
+    for k, p in enumerate(pts):
+        solo, _ = sweep.run_solo(p, steps, tile=tile)
+        assert len(solo) == steps
+        for c in range(steps):
+            np.testing.assert_array_equal(rasters[k][c], solo[c], err_msg=f"network {k} cycle {c}")
+        total += sum(len(x) for x in solo)
+    assert total > 0
+
+
+@pytest.mark.gpu
+def test_sweep_network_alone_matches_oracle():
+    steps = 250
+    for p in sweep.sample(2, seed=13, **SMALL):
+        net = api.Network()
+        net.add_chain(**p.kwargs())
+        spec = net.export_spec()
+        solo, _ = sweep.run_solo(p, steps)
+        o = oracle.run(spec, DT, steps)
+        np.testing.assert_array_equal(np.array([len(x) for x in solo]), o.raster_count)
+        np.testing.assert_array_equal(np.concatenate(solo), o.raster_ids)
