@@ -14,7 +14,7 @@ import sys
 HERE = os.path.dirname(os.path.abspath(__file__))
 CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libbsim_amd.so")
-SOURCES = ["step.hip", "capi.cpp", "network.cpp", "gen.cpp"]
+SOURCES = ["step.hip", "mlp.hip", "capi.cpp", "network.cpp", "gen.cpp"]
 
 
 def hipcc() -> str:

@@ -1,0 +1,756 @@
+/*
+ * mlp.hip -- the rate surrogate: a ReLU multilayer perceptron trained and run
+ * on the GPU with bf16 MFMA (v_mfma_f32_16x16x32_bf16, fp32 accumulation).
+ *
+ * Model: in (padded to 64) -> L hidden layers of H units (ReLU) -> out
+ * (padded to a multiple of 64, linear).  Weights are held twice: an fp32
+ * master copy updated by Adam and the bf16 copy the GEMMs read.  Activations
+ * and activation gradients are stored in bf16 (round to nearest even, done
+ * here bit by bit), every product is accumulated in fp32.
+ *
+ * One training step (bsim_mlp_train):
+ *   gather     minibatch rows of the resident data set -> X (bf16), Y (f32)
+ *   forward    H_l = bf16(relu(H_{l-1} W_l^T + b_l)),  out = H_L W^T + b (f32)
+ *   loss       dZ = bf16(2 (out - Y) / (B * out_dim)); loss and bias-gradient
+ *              partials per 64 rows
+ *   backward   dW_l = dZ_l^T H_{l-1} (split over the batch, fp32 partials);
+ *              dZ_{l-1} = bf16(dZ_l W_l) where H_{l-1} > 0, with its
+ *              bias-gradient partials per 64 rows
+ *   adam       sums every gradient's partials in a fixed order, then Adam
+ *              (Kingma & Ba, 2015) on the fp32 master, writes the bf16 copy
+ * Every reduction has a fixed order, so results repeat bit for bit.
+ *
+ * There is no counterpart in the reference; the algorithm is the textbook
+ * back-propagation of a fully connected ReLU network and Adam.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "../../include/bsim_mlp.h"
+#include "internal.h"
+
+typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
+typedef float f32x4 __attribute__((ext_vector_type(4)));
+
+namespace {
+
+constexpr int WAVE = 64;
+constexpr int BM = 64, BN = 64, BK = 32;   /* workgroup tile and k step */
+constexpr int LDS_STRIDE = BK + 8;         /* bf16 per LDS tile row (80 B) */
+constexpr int GEMM_THREADS = 256;          /* 2 x 2 wavefronts of 32 x 32 */
+constexpr int KSPLIT = 8;                  /* batch split of the weight gradients */
+constexpr int TRAIN_BATCH_ALIGN = KSPLIT * BK;   /* training batches: whole k steps per split */
+constexpr int ROWS_PER_PART = 64;          /* bias-gradient partial: rows per partial */
+constexpr int MAX_LAYERS = 10;             /* linear layers (hidden + 1) */
+
+__host__ __device__ inline uint16_t f2bf(float x)
+{
+    uint32_t u;
+    memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);   /* round to nearest even (finite x) */
+    return (uint16_t)(u >> 16);
+}
+
+__host__ __device__ inline float bf2f(uint16_t h)
+{
+    const uint32_t u = (uint32_t)h << 16;
+    float x;
+    memcpy(&x, &u, 4);
+    return x;
+}
+
+enum Epi { EPI_RELU_BF16 = 0, EPI_LINEAR_F32 = 1, EPI_DRELU_BF16 = 2, EPI_PARTIAL_F32 = 3 };
+
+/* C[m][n] = sum_k A(m, k) B(n, k) over this workgroup's k range.
+ * A(m, k) = AK ? A[m * lda + k] : A[k * lda + m]; B likewise with n. */
+struct GemmArgs {
+    const uint16_t *A;
+    const uint16_t *B;
+    int64_t lda, ldb;
+    int32_t M, N, K;        /* K: length of one split (blockIdx.z) */
+    int32_t pad;
+    const float *bias;      /* EPI_RELU_BF16, EPI_LINEAR_F32: [N] */
+    uint16_t *out_bf;       /* EPI_RELU_BF16, EPI_DRELU_BF16: [M][ldo] */
+    float *out_f;           /* EPI_LINEAR_F32: [M][ldo]; EPI_PARTIAL_F32: [split][M][ldo] */
+    int64_t ldo;
+    const uint16_t *mask;   /* EPI_DRELU_BF16: keep where mask[m * ldo + n] > 0 */
+    float *bpart;           /* EPI_DRELU_BF16: bias-gradient partials [M / 64][N] */
+};
+
+/* Global -> registers for one k step: each thread holds 8 bf16 of A and 8 of B. */
+template <bool KMAJ>
+__device__ inline uint4 load_tile(const uint16_t *P, int64_t ld, int32_t r0, int32_t k0, uint32_t tid)
+{
+    if (KMAJ) {   /* 64 rows x 32 k: row tid / 4, k (tid % 4) * 8 .. + 7 */
+        const uint32_t r = tid >> 2, kc = (tid & 3u) * 8u;
+        return *(const uint4 *)(P + (int64_t)(r0 + r) * ld + k0 + kc);
+    } else {      /* 32 k x 64 rows: k tid / 8, rows (tid % 8) * 8 .. + 7 */
+        const uint32_t k = tid >> 3, rc = (tid & 7u) * 8u;
+        return *(const uint4 *)(P + (int64_t)(k0 + k) * ld + r0 + rc);
+    }
+}
+
+/* Registers -> LDS tile [64 rows][LDS_STRIDE], k contiguous. */
+template <bool KMAJ>
+__device__ inline void store_tile(uint16_t *T, uint4 v, uint32_t tid)
+{
+    if (KMAJ) {
+        const uint32_t r = tid >> 2, kc = (tid & 3u) * 8u;
+        *(uint4 *)(T + r * LDS_STRIDE + kc) = v;
+    } else {
+        const uint32_t k = tid >> 3, rc = (tid & 7u) * 8u;
+        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
+#pragma unroll
+        for (int j = 0; j < 4; j++) {
+            T[(rc + 2 * j) * LDS_STRIDE + k] = (uint16_t)(w[j] & 0xFFFFu);
+            T[(rc + 2 * j + 1) * LDS_STRIDE + k] = (uint16_t)(w[j] >> 16);
+        }
+    }
+}
+
+template <bool AK, bool BKM, int EPI>
+__global__ __launch_bounds__(GEMM_THREADS) void mlp_gemm(GemmArgs g)
+{
+    __shared__ __attribute__((aligned(16))) uint16_t As[2][BM * LDS_STRIDE];
+    __shared__ __attribute__((aligned(16))) uint16_t Bs[2][BN * LDS_STRIDE];
+    __shared__ float bsum[2][BN];
+
+    const uint32_t tid = threadIdx.x;
+    const uint32_t lane = tid & (WAVE - 1), w = tid / WAVE;
+    const uint32_t wm = w >> 1, wn = w & 1u;     /* wavefront's 32 x 32 quadrant */
+    const int32_t n0 = blockIdx.x * BN, m0 = blockIdx.y * BM;
+    const int32_t kbeg = blockIdx.z * g.K;
+    const int32_t nk = g.K / BK;
+
+    f32x4 acc[2][2];
+#pragma unroll
+    for (int i = 0; i < 2; i++)
+#pragma unroll
+        for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
+
+    uint4 ra = load_tile<AK>(g.A, g.lda, m0, kbeg, tid);
+    uint4 rb = load_tile<BKM>(g.B, g.ldb, n0, kbeg, tid);
+    store_tile<AK>(As[0], ra, tid);
+    store_tile<BKM>(Bs[0], rb, tid);
+    __syncthreads();
+
+    const uint32_t r16 = lane & 15u, kg = (lane >> 4) * 8u;
+    for (int32_t s = 0; s < nk; s++) {
+        const int cur = s & 1;
+        if (s + 1 < nk) {   /* next k step in flight while this one multiplies */
+            ra = load_tile<AK>(g.A, g.lda, m0, kbeg + (s + 1) * BK, tid);
+            rb = load_tile<BKM>(g.B, g.ldb, n0, kbeg + (s + 1) * BK, tid);
+        }
+        bf16x8 a[2], b[2];
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+            a[i] = *(const bf16x8 *)(As[cur] + (wm * 32 + i * 16 + r16) * LDS_STRIDE + kg);
+#pragma unroll
+        for (int j = 0; j < 2; j++)
+            b[j] = *(const bf16x8 *)(Bs[cur] + (wn * 32 + j * 16 + r16) * LDS_STRIDE + kg);
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        if (s + 1 < nk) {
+            store_tile<AK>(As[cur ^ 1], ra, tid);
+            store_tile<BKM>(Bs[cur ^ 1], rb, tid);
+        }
+        __syncthreads();
+    }
+
+    /* lane holds C[row0 + 4 * (lane / 16) + r][col0 + lane % 16], r = 0..3 */
+    const uint32_t rq = (lane >> 4) * 4u;
+    float csum[2] = {0.f, 0.f};
+#pragma unroll
+    for (int i = 0; i < 2; i++) {
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            const int32_t col = n0 + wn * 32 + j * 16 + r16;
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int32_t row = m0 + wm * 32 + i * 16 + rq + r;
+                const float v = acc[i][j][r];
+                if (EPI == EPI_RELU_BF16) {
+                    const float z = v + g.bias[col];
+                    g.out_bf[(int64_t)row * g.ldo + col] = f2bf(z > 0.f ? z : 0.f);
+                } else if (EPI == EPI_LINEAR_F32) {
+                    g.out_f[(int64_t)row * g.ldo + col] = v + g.bias[col];
+                } else if (EPI == EPI_DRELU_BF16) {
+                    const uint16_t h = g.mask[(int64_t)row * g.ldo + col];
+                    const uint16_t dz = (h != 0 && !(h & 0x8000u)) ? f2bf(v) : (uint16_t)0;
+                    g.out_bf[(int64_t)row * g.ldo + col] = dz;
+                    csum[j] += bf2f(dz);
+                } else {
+                    g.out_f[((int64_t)blockIdx.z * g.M + row) * g.ldo + col] = v;
+                }
+            }
+        }
+    }
+    if (EPI == EPI_DRELU_BF16) {
+        /* column sums over the workgroup's 64 rows, in a fixed order */
+#pragma unroll
+        for (int j = 0; j < 2; j++) {
+            float x = csum[j];
+            x += __shfl_xor(x, 16);
+            x += __shfl_xor(x, 32);
+            if (lane < 16) bsum[wm][wn * 32 + j * 16 + lane] = x;
+        }
+        __syncthreads();
+        if (tid < BN) g.bpart[(int64_t)blockIdx.y * g.N + n0 + tid] = bsum[0][tid] + bsum[1][tid];
+    }
+}
+
+/* Loss of a minibatch: rows [64 * blockIdx.x, + 64), one thread per output
+ * column.  dZ = bf16(scale * (out - y)) (0 in padded columns); partial sums of
+ * (out - y)^2 and of the bf16 dZ per column. */
+__global__ __launch_bounds__(256) void mlp_loss(const float *out, const float *y, uint16_t *dz,
+                                                float *bpart, float *lpart, int32_t npad,
+                                                int32_t nvalid, float scale)
+{
+    __shared__ float red[256];
+    const int32_t r0 = blockIdx.x * ROWS_PER_PART;
+    float lsum = 0.f;
+    for (int32_t c = threadIdx.x; c < npad; c += blockDim.x) {
+        float cs = 0.f;
+        for (int32_t r = r0; r < r0 + ROWS_PER_PART; r++) {
+            uint16_t h = 0;
+            if (c < nvalid) {
+                const float diff = out[(int64_t)r * npad + c] - y[(int64_t)r * nvalid + c];
+                lsum += diff * diff;
+                h = f2bf(scale * diff);
+            }
+            dz[(int64_t)r * npad + c] = h;
+            cs += bf2f(h);
+        }
+        bpart[(int64_t)blockIdx.x * npad + c] = cs;
+    }
+    red[threadIdx.x] = lsum;
+    __syncthreads();
+    for (int s = 128; s > 0; s >>= 1) {
+        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) lpart[blockIdx.x] = red[0];
+}
+
+/* Minibatch gather: X rows (bf16, 64 wide) and Y rows of the resident set. */
+__global__ __launch_bounds__(256) void mlp_gather(const uint16_t *xall, const float *yall,
+                                                  const int32_t *rows, uint16_t *x, float *y,
+                                                  int32_t kin, int32_t nvalid)
+{
+    const int32_t b = blockIdx.x;
+    const int64_t src = rows[b];
+    for (int32_t c = threadIdx.x; c < kin; c += blockDim.x) x[(int64_t)b * kin + c] = xall[src * kin + c];
+    for (int32_t c = threadIdx.x; c < nvalid; c += blockDim.x)
+        y[(int64_t)b * nvalid + c] = yall[src * nvalid + c];
+}
+
+struct AdamSeg {
+    int64_t off, count;   /* into the flat parameter arrays */
+    const float *parts;   /* gradient partials, [nparts][count] */
+    int32_t nparts;
+    int32_t pad;
+};
+
+struct AdamArgs {
+    AdamSeg seg[2 * MAX_LAYERS];
+    int32_t nseg;
+    int32_t update;       /* 0: only write the summed gradient */
+    float lr, b1, b2, eps, c1, c2;   /* c1 = 1 - b1^t, c2 = 1 - b2^t */
+    float *p, *m, *v, *grad;
+    uint16_t *pbf;
+    const float *lpart;   /* loss partials of the step, summed by thread 0 */
+    int32_t nlpart;
+    float lscale;
+    float *loss;
+};
+
+/* Adam on the fp32 master (m, v in fp32), the bf16 copy written alongside. */
+__global__ __launch_bounds__(256) void mlp_adam(AdamArgs a)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0 && a.loss) {
+        float s = 0.f;
+        for (int32_t k = 0; k < a.nlpart; k++) s += a.lpart[k];
+        *a.loss = s * a.lscale;
+    }
+    int sidx = -1;
+    for (int s = 0; s < a.nseg; s++)
+        if (i >= a.seg[s].off && i < a.seg[s].off + a.seg[s].count) sidx = s;
+    if (sidx < 0) return;
+    const AdamSeg &sg = a.seg[sidx];
+    const int64_t j = i - sg.off;
+    float gsum = 0.f;
+    for (int32_t k = 0; k < sg.nparts; k++) gsum += sg.parts[(int64_t)k * sg.count + j];
+    if (a.grad) a.grad[i] = gsum;
+    if (!a.update) return;
+    const float m = a.b1 * a.m[i] + (1.f - a.b1) * gsum;
+    const float v = a.b2 * a.v[i] + (1.f - a.b2) * (gsum * gsum);
+    a.m[i] = m;
+    a.v[i] = v;
+    const float mhat = m / a.c1, vhat = v / a.c2;
+    const float p = a.p[i] - a.lr * (mhat / (sqrtf(vhat) + a.eps));
+    a.p[i] = p;
+    a.pbf[i] = f2bf(p);
+}
+
+/* Input conversion: f32 [n][in_dim] -> bf16 [n][kin] (zero padded). */
+__global__ __launch_bounds__(256) void mlp_convert_x(const float *x, uint16_t *xb, int64_t n,
+                                                     int32_t in_dim, int32_t kin)
+{
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n * kin) return;
+    const int64_t r = i / kin;
+    const int32_t c = (int32_t)(i % kin);
+    xb[i] = c < in_dim ? f2bf(x[r * in_dim + c]) : (uint16_t)0;
+}
+
+template <bool AK, bool BKM, int EPI>
+hipError_t gemm(const GemmArgs &g, int32_t splits, hipStream_t st)
+{
+    const dim3 grid(g.N / BN, g.M / BM, splits);
+    hipLaunchKernelGGL((mlp_gemm<AK, BKM, EPI>), grid, dim3(GEMM_THREADS), 0, st, g);
+    return hipGetLastError();
+}
+
+int64_t round_up(int64_t x, int64_t m) { return (x + m - 1) / m * m; }
+
+}  // namespace
+
+struct bsim_mlp {
+    bsim_mlp_config cfg{};
+    int32_t nlin = 0;                 /* linear layers */
+    int32_t kdim[MAX_LAYERS + 1]{};   /* padded widths: kdim[0] input, kdim[nlin] output */
+    int64_t woff[MAX_LAYERS]{}, boff[MAX_LAYERS]{};
+    int64_t nparam = 0;
+    int32_t step = 0;                 /* Adam steps taken */
+    hipStream_t stream = nullptr;
+    std::vector<void *> allocs;
+    float *p = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
+    uint16_t *pbf = nullptr;
+    uint16_t *act[MAX_LAYERS + 1]{};  /* act[0] input, act[l] hidden l (bf16) */
+    float *out = nullptr, *ybat = nullptr;
+    uint16_t *dz[2]{};                /* activation gradients (ping-pong) */
+    float *wpart[MAX_LAYERS]{}, *bpart[MAX_LAYERS]{};
+    float *lpart = nullptr, *loss = nullptr;
+    uint16_t *xall = nullptr;
+    float *yall = nullptr;
+    int64_t ndata = 0;
+    float *xstage = nullptr;          /* predict input staging (f32) */
+};
+
+namespace {
+
+int mfail(int code, const std::string &msg)
+{
+    g_bsim_err = msg;
+    return code;
+}
+
+int hip_check(hipError_t e, const char *what)
+{
+    if (e == hipSuccess) return BSIM_OK;
+    return mfail(BSIM_E_HIP, std::string(what) + ": " + hipGetErrorString(e));
+}
+
+template <typename T>
+int dalloc(bsim_mlp *m, T **p, int64_t count)
+{
+    void *q = nullptr;
+    const hipError_t e = hipMalloc(&q, (size_t)std::max<int64_t>(count, 1) * sizeof(T));
+    if (e != hipSuccess) return hip_check(e, "hipMalloc");
+    m->allocs.push_back(q);
+    *p = (T *)q;
+    return BSIM_OK;
+}
+
+/* uniform in [-1, 1) from (seed, index) */
+double hash_uniform(uint64_t seed, uint64_t i)
+{
+    uint64_t z = seed + 0x9E3779B97F4A7C15ull * (i + 1);
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
+}
+
+/* forward pass on act[0] (batch rows); fp32 output in m->out */
+int forward(bsim_mlp *m, int32_t batch)
+{
+    for (int l = 0; l < m->nlin; l++) {
+        GemmArgs g{};
+        g.A = m->act[l];
+        g.lda = m->kdim[l];
+        g.B = m->pbf + m->woff[l];
+        g.ldb = m->kdim[l];
+        g.M = batch;
+        g.N = m->kdim[l + 1];
+        g.K = m->kdim[l];
+        g.bias = m->p + m->boff[l];
+        g.ldo = m->kdim[l + 1];
+        hipError_t e;
+        if (l + 1 < m->nlin) {
+            g.out_bf = m->act[l + 1];
+            e = gemm<true, true, EPI_RELU_BF16>(g, 1, m->stream);
+        } else {
+            g.out_f = m->out;
+            e = gemm<true, true, EPI_LINEAR_F32>(g, 1, m->stream);
+        }
+        if (e != hipSuccess) return hip_check(e, "forward launch");
+    }
+    return BSIM_OK;
+}
+
+/* loss + backward on a batch whose act[0] and ybat are set; gradient
+ * partials left in wpart / bpart */
+int backward(bsim_mlp *m, int32_t batch)
+{
+    const int L = m->nlin;
+    const int32_t nout = m->kdim[L];
+    const float scale = 2.0f / ((float)batch * (float)m->cfg.out_dim);
+    hipLaunchKernelGGL(mlp_loss, dim3(batch / ROWS_PER_PART), dim3(256), 0, m->stream, m->out,
+                       m->ybat, m->dz[0], m->bpart[L - 1], m->lpart, nout, m->cfg.out_dim, scale);
+    hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_check(e, "loss launch");
+    int cur = 0;
+    for (int l = L - 1; l >= 0; l--) {
+        /* dW_l [kdim[l+1]][kdim[l]] = dZ^T H_{l}: A(m = out unit, k = row), B(n = in unit, k = row) */
+        GemmArgs gw{};
+        gw.A = m->dz[cur];
+        gw.lda = m->kdim[l + 1];
+        gw.B = m->act[l];
+        gw.ldb = m->kdim[l];
+        gw.M = m->kdim[l + 1];
+        gw.N = m->kdim[l];
+        gw.K = batch / KSPLIT;
+        gw.out_f = m->wpart[l];
+        gw.ldo = m->kdim[l];
+        e = gemm<false, false, EPI_PARTIAL_F32>(gw, KSPLIT, m->stream);
+        if (e != hipSuccess) return hip_check(e, "weight-gradient launch");
+        if (l == 0) break;
+        /* dZ_{l-1} [batch][kdim[l]] = (dZ W_l) where H > 0: B(n = in unit, k = out unit) = W[k][n] */
+        GemmArgs gx{};
+        gx.A = m->dz[cur];
+        gx.lda = m->kdim[l + 1];
+        gx.B = m->pbf + m->woff[l];
+        gx.ldb = m->kdim[l];
+        gx.M = batch;
+        gx.N = m->kdim[l];
+        gx.K = m->kdim[l + 1];
+        gx.out_bf = m->dz[cur ^ 1];
+        gx.ldo = m->kdim[l];
+        gx.mask = m->act[l];
+        gx.bpart = m->bpart[l - 1];
+        e = gemm<true, false, EPI_DRELU_BF16>(gx, 1, m->stream);
+        if (e != hipSuccess) return hip_check(e, "activation-gradient launch");
+        cur ^= 1;
+    }
+    return BSIM_OK;
+}
+
+int adam(bsim_mlp *m, int32_t batch, bool update, float lr, float *grad_out, float *loss_dev)
+{
+    AdamArgs a{};
+    a.nseg = 2 * m->nlin;
+    for (int l = 0; l < m->nlin; l++) {
+        a.seg[2 * l] = AdamSeg{m->woff[l], (int64_t)m->kdim[l + 1] * m->kdim[l], m->wpart[l], KSPLIT, 0};
+        a.seg[2 * l + 1] = AdamSeg{m->boff[l], m->kdim[l + 1], m->bpart[l], batch / ROWS_PER_PART, 0};
+    }
+    a.update = update ? 1 : 0;
+    a.lr = lr;
+    a.b1 = m->cfg.beta1;
+    a.b2 = m->cfg.beta2;
+    a.eps = m->cfg.eps;
+    const int t = m->step + 1;
+    a.c1 = (float)(1.0 - std::pow((double)m->cfg.beta1, t));
+    a.c2 = (float)(1.0 - std::pow((double)m->cfg.beta2, t));
+    a.p = m->p;
+    a.m = m->m;
+    a.v = m->v;
+    a.grad = grad_out;
+    a.pbf = m->pbf;
+    a.lpart = m->lpart;
+    a.nlpart = batch / ROWS_PER_PART;
+    a.lscale = 1.0f / ((float)batch * (float)m->cfg.out_dim);
+    a.loss = loss_dev;
+    const int64_t blocks = (m->nparam + 255) / 256;
+    hipLaunchKernelGGL(mlp_adam, dim3((uint32_t)blocks), dim3(256), 0, m->stream, a);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) return hip_check(e, "adam launch");
+    if (update) m->step++;
+    return BSIM_OK;
+}
+
+int check_batch(const bsim_mlp *m, int32_t batch)
+{
+    if (batch <= 0 || batch % TRAIN_BATCH_ALIGN != 0 || batch > m->cfg.max_batch)
+        return mfail(BSIM_E_ARG, "batch must be a positive multiple of " +
+                                     std::to_string(TRAIN_BATCH_ALIGN) + " and at most max_batch (" +
+                                     std::to_string(m->cfg.max_batch) + ")");
+    return BSIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+bsim_mlp *bsim_mlp_create(const bsim_mlp_config *cfg)
+{
+    if (!cfg) {
+        g_bsim_err = "null config";
+        return nullptr;
+    }
+    const bsim_mlp_config c = *cfg;
+    if (c.in_dim < 1 || c.in_dim > 64 || c.hidden < 64 || c.hidden % 64 != 0 || c.hidden > 4096 ||
+        c.layers < 1 || c.layers > MAX_LAYERS - 1 || c.out_dim < 1 || c.out_dim > 4096 ||
+        c.max_batch < TRAIN_BATCH_ALIGN || c.max_batch % TRAIN_BATCH_ALIGN != 0 ||
+        c.max_batch > (1 << 20) || !(c.beta1 >= 0.f && c.beta1 < 1.f) ||
+        !(c.beta2 >= 0.f && c.beta2 < 1.f) || !(c.eps > 0.f)) {
+        g_bsim_err = "invalid MLP config: in_dim 1..64, hidden a multiple of 64 up to 4096, "
+                     "layers 1..9, out_dim 1..4096, max_batch a multiple of 256 up to 2^20, "
+                     "0 <= beta < 1, eps > 0";
+        return nullptr;
+    }
+    bsim_mlp *m = new (std::nothrow) bsim_mlp();
+    if (!m) {
+        g_bsim_err = "out of host memory";
+        return nullptr;
+    }
+    m->cfg = c;
+    m->nlin = c.layers + 1;
+    m->kdim[0] = 64;
+    for (int l = 1; l < m->nlin; l++) m->kdim[l] = c.hidden;
+    m->kdim[m->nlin] = (int32_t)round_up(c.out_dim, 64);
+    int64_t off = 0;
+    for (int l = 0; l < m->nlin; l++) {
+        m->woff[l] = off;
+        off += (int64_t)m->kdim[l + 1] * m->kdim[l];
+        m->boff[l] = off;
+        off += m->kdim[l + 1];
+    }
+    m->nparam = off;
+    auto bad = [&](int) {
+        bsim_mlp_destroy(m);
+        return (bsim_mlp *)nullptr;
+    };
+    if (hip_check(hipSetDevice(c.device), "hipSetDevice") != BSIM_OK) return bad(0);
+    if (hip_check(hipStreamCreateWithFlags(&m->stream, hipStreamNonBlocking), "stream") != BSIM_OK)
+        return bad(0);
+    const int64_t B = c.max_batch;
+    int rc = dalloc(m, &m->p, m->nparam);
+    if (!rc) rc = dalloc(m, &m->m, m->nparam);
+    if (!rc) rc = dalloc(m, &m->v, m->nparam);
+    if (!rc) rc = dalloc(m, &m->grad, m->nparam);
+    if (!rc) rc = dalloc(m, &m->pbf, m->nparam);
+    for (int l = 0; l < m->nlin && !rc; l++) rc = dalloc(m, &m->act[l], B * m->kdim[l]);
+    if (!rc) rc = dalloc(m, &m->out, B * m->kdim[m->nlin]);
+    if (!rc) rc = dalloc(m, &m->ybat, B * c.out_dim);
+    int32_t wmax = 0;
+    for (int l = 0; l <= m->nlin; l++) wmax = std::max(wmax, m->kdim[l]);
+    if (!rc) rc = dalloc(m, &m->dz[0], B * wmax);
+    if (!rc) rc = dalloc(m, &m->dz[1], B * wmax);
+    for (int l = 0; l < m->nlin && !rc; l++) {
+        rc = dalloc(m, &m->wpart[l], (int64_t)KSPLIT * m->kdim[l + 1] * m->kdim[l]);
+        if (!rc) rc = dalloc(m, &m->bpart[l], B / ROWS_PER_PART * m->kdim[l + 1]);
+    }
+    if (!rc) rc = dalloc(m, &m->lpart, B / ROWS_PER_PART);
+    if (!rc) rc = dalloc(m, &m->loss, 1);
+    if (!rc) rc = dalloc(m, &m->xstage, B * c.in_dim);
+    if (rc != BSIM_OK) return bad(0);
+    /* He-uniform weights (bound sqrt(6 / fan_in)), zero biases, from the seed */
+    std::vector<float> init(m->nparam, 0.f);
+    for (int l = 0; l < m->nlin; l++) {
+        const int32_t fan_in = l == 0 ? c.in_dim : c.hidden;
+        const int32_t nrow = l + 1 == m->nlin ? c.out_dim : c.hidden;
+        const double bound = std::sqrt(6.0 / fan_in);
+        for (int32_t r = 0; r < nrow; r++)
+            for (int32_t k = 0; k < fan_in; k++) {
+                const int64_t i = m->woff[l] + (int64_t)r * m->kdim[l] + k;
+                init[i] = (float)(bound * hash_uniform(c.seed, (uint64_t)i));
+            }
+    }
+    if (bsim_mlp_set_params(m, init.data()) != BSIM_OK) return bad(0);
+    return m;
+}
+
+void bsim_mlp_destroy(bsim_mlp *m)
+{
+    if (!m) return;
+    if (m->stream) hipStreamSynchronize(m->stream);
+    for (void *q : m->allocs) hipFree(q);
+    if (m->stream) hipStreamDestroy(m->stream);
+    delete m;
+}
+
+int64_t bsim_mlp_num_params(const bsim_mlp *m) { return m ? m->nparam : -1; }
+
+int bsim_mlp_layout(const bsim_mlp *m, int32_t *dims, int32_t cap)
+{
+    if (!m || !dims) return mfail(BSIM_E_ARG, "null argument");
+    if (cap < m->nlin + 1) return mfail(BSIM_E_ARG, "dims capacity too small");
+    for (int l = 0; l <= m->nlin; l++) dims[l] = m->kdim[l];
+    return m->nlin;
+}
+
+int bsim_mlp_get_params(bsim_mlp *m, float *out)
+{
+    if (!m || !out) return mfail(BSIM_E_ARG, "null argument");
+    if (int rc = hip_check(hipStreamSynchronize(m->stream), "sync")) return rc;
+    return hip_check(hipMemcpy(out, m->p, m->nparam * sizeof(float), hipMemcpyDeviceToHost),
+                     "hipMemcpy");
+}
+
+int bsim_mlp_set_params(bsim_mlp *m, const float *in)
+{
+    if (!m || !in) return mfail(BSIM_E_ARG, "null argument");
+    if (int rc = hip_check(hipStreamSynchronize(m->stream), "sync")) return rc;
+    std::vector<float> p(in, in + m->nparam);
+    /* padded positions stay zero */
+    for (int l = 0; l < m->nlin; l++) {
+        const int32_t fan_in = l == 0 ? m->cfg.in_dim : m->cfg.hidden;
+        const int32_t nrow = l + 1 == m->nlin ? m->cfg.out_dim : m->cfg.hidden;
+        for (int32_t r = 0; r < m->kdim[l + 1]; r++) {
+            for (int32_t k = 0; k < m->kdim[l]; k++)
+                if (r >= nrow || k >= fan_in) p[m->woff[l] + (int64_t)r * m->kdim[l] + k] = 0.f;
+            if (r >= nrow) p[m->boff[l] + r] = 0.f;
+        }
+    }
+    std::vector<uint16_t> pb(m->nparam);
+    for (int64_t i = 0; i < m->nparam; i++) pb[i] = f2bf(p[i]);
+    int rc = hip_check(hipMemcpy(m->p, p.data(), m->nparam * 4, hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = hip_check(hipMemcpy(m->pbf, pb.data(), m->nparam * 2, hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = hip_check(hipMemset(m->m, 0, m->nparam * 4), "hipMemset");
+    if (!rc) rc = hip_check(hipMemset(m->v, 0, m->nparam * 4), "hipMemset");
+    m->step = 0;
+    return rc;
+}
+
+int bsim_mlp_set_data(bsim_mlp *m, const float *x, const float *y, int64_t n)
+{
+    if (!m || !x || !y || n <= 0) return mfail(BSIM_E_ARG, "null argument or empty data set");
+    if (int rc = hip_check(hipStreamSynchronize(m->stream), "sync")) return rc;
+    if (m->xall) {
+        for (auto it = m->allocs.begin(); it != m->allocs.end();)
+            if (*it == m->xall || *it == m->yall) {
+                hipFree(*it);
+                it = m->allocs.erase(it);
+            } else {
+                ++it;
+            }
+        m->xall = nullptr;
+        m->yall = nullptr;
+        m->ndata = 0;
+    }
+    const int32_t kin = m->kdim[0], din = m->cfg.in_dim, dout = m->cfg.out_dim;
+    if (int rc = dalloc(m, &m->xall, n * kin)) return rc;
+    if (int rc = dalloc(m, &m->yall, n * dout)) return rc;
+    std::vector<uint16_t> xb((size_t)(n * kin), 0);
+    for (int64_t r = 0; r < n; r++)
+        for (int32_t c = 0; c < din; c++) xb[r * kin + c] = f2bf(x[r * din + c]);
+    int rc = hip_check(hipMemcpy(m->xall, xb.data(), xb.size() * 2, hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = hip_check(hipMemcpy(m->yall, y, (size_t)(n * dout) * 4, hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) m->ndata = n;
+    return rc;
+}
+
+int bsim_mlp_train(bsim_mlp *m, const int32_t *rows, int32_t steps, int32_t batch, float lr,
+                   float *loss)
+{
+    if (!m || !rows || steps < 0) return mfail(BSIM_E_ARG, "null argument");
+    if (int rc = check_batch(m, batch)) return rc;
+    if (m->ndata <= 0) return mfail(BSIM_E_ARG, "no data set (bsim_mlp_set_data)");
+    for (int64_t i = 0; i < (int64_t)steps * batch; i++)
+        if (rows[i] < 0 || rows[i] >= m->ndata) return mfail(BSIM_E_ARG, "row index out of range");
+    if (steps == 0) return BSIM_OK;
+    float *ldev = nullptr;
+    int32_t *rdev = nullptr;
+    int rc = dalloc(m, &ldev, steps);
+    if (!rc) rc = dalloc(m, &rdev, (int64_t)steps * batch);
+    if (!rc) rc = hip_check(hipMemcpy(rdev, rows, (size_t)steps * batch * 4, hipMemcpyHostToDevice),
+                            "hipMemcpy");
+    for (int32_t s = 0; s < steps && !rc; s++) {
+        hipLaunchKernelGGL(mlp_gather, dim3(batch), dim3(256), 0, m->stream, m->xall, m->yall,
+                           rdev + (int64_t)s * batch, m->act[0], m->ybat, m->kdim[0],
+                           m->cfg.out_dim);
+        rc = hip_check(hipGetLastError(), "gather launch");
+        if (!rc) rc = forward(m, batch);
+        if (!rc) rc = backward(m, batch);
+        if (!rc) rc = adam(m, batch, true, lr, nullptr, ldev + s);
+    }
+    const int src = hip_check(hipStreamSynchronize(m->stream), "sync");
+    if (!rc) rc = src;
+    if (!rc && loss)
+        rc = hip_check(hipMemcpy(loss, ldev, steps * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    for (void *q : {(void *)ldev, (void *)rdev}) {
+        if (!q) continue;
+        m->allocs.erase(std::find(m->allocs.begin(), m->allocs.end(), q));
+        hipFree(q);
+    }
+    return rc;
+}
+
+int bsim_mlp_gradients(bsim_mlp *m, const float *x, const float *y, int32_t batch, float *grad,
+                       float *loss)
+{
+    if (!m || !x || !y || !grad || !loss) return mfail(BSIM_E_ARG, "null argument");
+    if (int rc = check_batch(m, batch)) return rc;
+    const int32_t din = m->cfg.in_dim, kin = m->kdim[0];
+    int rc = hip_check(hipMemcpyAsync(m->xstage, x, (size_t)batch * din * 4, hipMemcpyHostToDevice,
+                                      m->stream), "hipMemcpyAsync");
+    if (!rc) rc = hip_check(hipMemcpyAsync(m->ybat, y, (size_t)batch * m->cfg.out_dim * 4,
+                                           hipMemcpyHostToDevice, m->stream), "hipMemcpyAsync");
+    if (rc) return rc;
+    const int64_t tot = (int64_t)batch * kin;
+    hipLaunchKernelGGL(mlp_convert_x, dim3((uint32_t)((tot + 255) / 256)), dim3(256), 0, m->stream,
+                       m->xstage, m->act[0], (int64_t)batch, din, kin);
+    rc = hip_check(hipGetLastError(), "convert launch");
+    if (!rc) rc = forward(m, batch);
+    if (!rc) rc = backward(m, batch);
+    if (!rc) rc = adam(m, batch, false, 0.f, m->grad, m->loss);
+    if (!rc) rc = hip_check(hipStreamSynchronize(m->stream), "sync");
+    if (!rc) rc = hip_check(hipMemcpy(grad, m->grad, m->nparam * 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    if (!rc) rc = hip_check(hipMemcpy(loss, m->loss, 4, hipMemcpyDeviceToHost), "hipMemcpy");
+    return rc;
+}
+
+int bsim_mlp_predict(bsim_mlp *m, const float *x, int64_t n, float *y)
+{
+    if (!m || !x || !y || n < 0) return mfail(BSIM_E_ARG, "null argument");
+    const int32_t din = m->cfg.in_dim, kin = m->kdim[0], dout = m->cfg.out_dim;
+    const int32_t nout = m->kdim[m->nlin];
+    const int64_t B = m->cfg.max_batch;
+    std::vector<float> tmp;
+    int rc = BSIM_OK;
+    for (int64_t r0 = 0; r0 < n && !rc; r0 += B) {
+        const int64_t rows = std::min<int64_t>(B, n - r0);
+        const int32_t batch = (int32_t)round_up(rows, 64);
+        rc = hip_check(hipMemsetAsync(m->xstage, 0, (size_t)batch * din * 4, m->stream), "memset");
+        if (!rc) rc = hip_check(hipMemcpyAsync(m->xstage, x + r0 * din, (size_t)rows * din * 4,
+                                               hipMemcpyHostToDevice, m->stream), "hipMemcpyAsync");
+        if (rc) break;
+        const int64_t tot = (int64_t)batch * kin;
+        hipLaunchKernelGGL(mlp_convert_x, dim3((uint32_t)((tot + 255) / 256)), dim3(256), 0,
+                           m->stream, m->xstage, m->act[0], (int64_t)batch, din, kin);
+        rc = hip_check(hipGetLastError(), "convert launch");
+        if (!rc) rc = forward(m, batch);
+        if (rc) break;
+        tmp.resize((size_t)rows * nout);
+        rc = hip_check(hipMemcpyAsync(tmp.data(), m->out, (size_t)rows * nout * 4,
+                                      hipMemcpyDeviceToHost, m->stream), "hipMemcpyAsync");
+        if (!rc) rc = hip_check(hipStreamSynchronize(m->stream), "sync");
+        for (int64_t r = 0; r < rows && !rc; r++)
+            memcpy(y + (r0 + r) * dout, tmp.data() + r * nout, dout * 4);
+    }
+    return rc;
+}
+
+}  // extern "C"

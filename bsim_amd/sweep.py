@@ -57,25 +57,65 @@ class Point:
                          float(np.mean(self.sizes))], np.float64)
 
 
-def sample(n_nets, seed=0, npop=C3["npop"], size=C3["size"], fan_in=C3["fan_in"], w0=C3["w0"],
-           inh_frac=C3["inh_frac"], drive_hz=C3["drive_hz"], dmax=C3["dmax"], n_drive=None,
-           drive_fan_in=50, drive_w=2.0):
-    """n_nets sweep points drawn from the given ranges (seeded)."""
+@dataclass
+class Table:
+    """n sweep points as arrays (one row per point; sizes zero past npop)."""
+    npop: np.ndarray
+    sizes: np.ndarray
+    fan_in: np.ndarray
+    w0: np.ndarray
+    inh_frac: np.ndarray
+    dmax: np.ndarray
+    drive_hz: np.ndarray
+    n_drive: np.ndarray
+    seed: np.ndarray
+    drive_fan_in: int
+    drive_w: float
+
+    def __len__(self):
+        return len(self.npop)
+
+    def point(self, i):
+        k = int(self.npop[i])
+        return Point(sizes=[int(x) for x in self.sizes[i, :k]], fan_in=int(self.fan_in[i]),
+                     w0=float(self.w0[i]), inh_frac=float(self.inh_frac[i]),
+                     dmax=int(self.dmax[i]), drive_rate_hz=float(self.drive_hz[i]),
+                     n_drive=int(self.n_drive[i]), drive_fan_in=self.drive_fan_in,
+                     drive_w=self.drive_w, seed=int(self.seed[i]))
+
+    def points(self, idx=None):
+        idx = range(len(self)) if idx is None else idx
+        return [self.point(int(i)) for i in idx]
+
+
+def sample_table(n_nets, seed=0, npop=C3["npop"], size=C3["size"], fan_in=C3["fan_in"],
+                 w0=C3["w0"], inh_frac=C3["inh_frac"], drive_hz=C3["drive_hz"], dmax=C3["dmax"],
+                 n_drive=None, drive_fan_in=50, drive_w=2.0):
+    """n_nets sweep points drawn from the given ranges (seeded): population
+    count and maximum delay uniform integers, sizes log-uniform, fan-in
+    uniform integer, w0 / inhibitory fraction / drive rate uniform (fp32);
+    the drive population has the first population's size unless n_drive is
+    given."""
     rng = np.random.default_rng(seed)
-    pts = []
-    for i in range(n_nets):
-        k = int(rng.integers(npop[0], npop[1] + 1))
-        sizes = np.exp(rng.uniform(np.log(size[0]), np.log(size[1]), k)).astype(np.int64)
-        sizes = [int(max(1, x)) for x in sizes]
-        nd = sizes[0] if n_drive is None else int(n_drive)
-        pts.append(Point(sizes=sizes, fan_in=int(rng.integers(fan_in[0], fan_in[1] + 1)),
-                         w0=float(np.float32(rng.uniform(*w0))),
-                         inh_frac=float(np.float32(rng.uniform(*inh_frac))),
-                         dmax=int(rng.integers(dmax[0], dmax[1] + 1)),
-                         drive_rate_hz=float(np.float32(rng.uniform(*drive_hz))),
-                         n_drive=nd, drive_fan_in=int(drive_fan_in), drive_w=float(drive_w),
-                         seed=int(rng.integers(1, 2**62))))
-    return pts
+    n = int(n_nets)
+    k = rng.integers(npop[0], npop[1] + 1, n)
+    sizes = np.exp(rng.uniform(np.log(size[0]), np.log(size[1]), (n, 8))).astype(np.int64)
+    sizes = np.maximum(sizes, 1)
+    sizes[np.arange(8)[None, :] >= k[:, None]] = 0
+    fi = rng.integers(fan_in[0], fan_in[1] + 1, n)
+    w = rng.uniform(w0[0], w0[1], n).astype(np.float32)
+    inh = rng.uniform(inh_frac[0], inh_frac[1], n).astype(np.float32)
+    dm = rng.integers(dmax[0], dmax[1] + 1, n)
+    dr = rng.uniform(drive_hz[0], drive_hz[1], n).astype(np.float32)
+    sd = rng.integers(1, 2 ** 62, n)
+    nd = sizes[:, 0].copy() if n_drive is None else np.full(n, int(n_drive), np.int64)
+    return Table(npop=k, sizes=sizes, fan_in=fi, w0=w, inh_frac=inh, dmax=dm, drive_hz=dr,
+                 n_drive=nd, seed=sd, drive_fan_in=int(drive_fan_in), drive_w=float(drive_w))
+
+
+def sample(n_nets, seed=0, **ranges):
+    """n_nets sweep points (sample_table's draws) as Point objects."""
+    return sample_table(n_nets, seed, **ranges).points()
 
 
 @dataclass
