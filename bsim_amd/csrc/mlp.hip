@@ -16,6 +16,12 @@
  *   backward   dW_l = dZ_l^T H_{l-1} (split over the batch, fp32 partials);
  *              dZ_{l-1} = bf16(dZ_l W_l) where H_{l-1} > 0, with its
  *              bias-gradient partials per 64 rows
+ * The input, hidden activations and activation gradients of a training step
+ * are also written transposed ([unit][row]), so that the weight-gradient
+ * GEMM reads both of its operands with the batch (its reduction index)
+ * contiguous; likewise a transposed bf16 copy of every weight matrix, renewed
+ * after each update, lets the activation-gradient GEMM read W^T with its
+ * reduction index contiguous.
  *   adam       sums every gradient's partials in a fixed order, then Adam
  *              (Kingma & Ba, 2015) on the fp32 master, writes the bf16 copy
  * Every reduction has a fixed order, so results repeat bit for bit.
@@ -42,10 +48,10 @@ typedef float f32x4 __attribute__((ext_vector_type(4)));
 namespace {
 
 constexpr int WAVE = 64;
-constexpr int BM = 64, BN = 64, BK = 32;   /* workgroup tile and k step */
-constexpr int LDS_STRIDE = BK + 8;         /* bf16 per LDS tile row (80 B) */
+constexpr int BM = 64, BN = 64, BK = 64;   /* workgroup tile and k step */
+constexpr int LDS_STRIDE = BK + 8;         /* bf16 per LDS tile row (144 B) */
 constexpr int GEMM_THREADS = 256;          /* 2 x 2 wavefronts of 32 x 32 */
-constexpr int KSPLIT = 8;                  /* batch split of the weight gradients */
+constexpr int KSPLIT = 4;                  /* batch split of the weight gradients */
 constexpr int TRAIN_BATCH_ALIGN = KSPLIT * BK;   /* training batches: whole k steps per split */
 constexpr int ROWS_PER_PART = 64;          /* bias-gradient partial: rows per partial */
 constexpr int MAX_LAYERS = 10;             /* linear layers (hidden + 1) */
@@ -68,8 +74,8 @@ __host__ __device__ inline float bf2f(uint16_t h)
 
 enum Epi { EPI_RELU_BF16 = 0, EPI_LINEAR_F32 = 1, EPI_DRELU_BF16 = 2, EPI_PARTIAL_F32 = 3 };
 
-/* C[m][n] = sum_k A(m, k) B(n, k) over this workgroup's k range.
- * A(m, k) = AK ? A[m * lda + k] : A[k * lda + m]; B likewise with n. */
+/* C[m][n] = sum_k A[m * lda + k] B[n * ldb + k] over this workgroup's k
+ * range (both operands k-contiguous). */
 struct GemmArgs {
     const uint16_t *A;
     const uint16_t *B;
@@ -82,40 +88,33 @@ struct GemmArgs {
     int64_t ldo;
     const uint16_t *mask;   /* EPI_DRELU_BF16: keep where mask[m * ldo + n] > 0 */
     float *bpart;           /* EPI_DRELU_BF16: bias-gradient partials [M / 64][N] */
+    uint16_t *out_t;        /* EPI_RELU_BF16, EPI_DRELU_BF16: transposed copy [N][ldt] or null */
+    int64_t ldt;
 };
 
-/* Global -> registers for one k step: each thread holds 8 bf16 of A and 8 of B. */
-template <bool KMAJ>
-__device__ inline uint4 load_tile(const uint16_t *P, int64_t ld, int32_t r0, int32_t k0, uint32_t tid)
+/* One k step of a 64-row operand: thread tid holds rows tid / 8 and
+ * tid / 8 + 32, k (tid % 8) * 8 .. + 7. */
+struct Tile {
+    uint4 v[2];
+};
+
+__device__ inline Tile load_tile(const uint16_t *P, int64_t ld, int32_t r0, int32_t k0, uint32_t tid)
 {
-    if (KMAJ) {   /* 64 rows x 32 k: row tid / 4, k (tid % 4) * 8 .. + 7 */
-        const uint32_t r = tid >> 2, kc = (tid & 3u) * 8u;
-        return *(const uint4 *)(P + (int64_t)(r0 + r) * ld + k0 + kc);
-    } else {      /* 32 k x 64 rows: k tid / 8, rows (tid % 8) * 8 .. + 7 */
-        const uint32_t k = tid >> 3, rc = (tid & 7u) * 8u;
-        return *(const uint4 *)(P + (int64_t)(k0 + k) * ld + r0 + rc);
-    }
+    const uint32_t r = tid >> 3, kc = (tid & 7u) * 8u;
+    Tile t;
+    t.v[0] = *(const uint4 *)(P + (int64_t)(r0 + r) * ld + k0 + kc);
+    t.v[1] = *(const uint4 *)(P + (int64_t)(r0 + r + 32) * ld + k0 + kc);
+    return t;
 }
 
-/* Registers -> LDS tile [64 rows][LDS_STRIDE], k contiguous. */
-template <bool KMAJ>
-__device__ inline void store_tile(uint16_t *T, uint4 v, uint32_t tid)
+__device__ inline void store_tile(uint16_t *T, const Tile &t, uint32_t tid)
 {
-    if (KMAJ) {
-        const uint32_t r = tid >> 2, kc = (tid & 3u) * 8u;
-        *(uint4 *)(T + r * LDS_STRIDE + kc) = v;
-    } else {
-        const uint32_t k = tid >> 3, rc = (tid & 7u) * 8u;
-        const uint32_t w[4] = {v.x, v.y, v.z, v.w};
-#pragma unroll
-        for (int j = 0; j < 4; j++) {
-            T[(rc + 2 * j) * LDS_STRIDE + k] = (uint16_t)(w[j] & 0xFFFFu);
-            T[(rc + 2 * j + 1) * LDS_STRIDE + k] = (uint16_t)(w[j] >> 16);
-        }
-    }
+    const uint32_t r = tid >> 3, kc = (tid & 7u) * 8u;
+    *(uint4 *)(T + r * LDS_STRIDE + kc) = t.v[0];
+    *(uint4 *)(T + (r + 32) * LDS_STRIDE + kc) = t.v[1];
 }
 
-template <bool AK, bool BKM, int EPI>
+template <int EPI>
 __global__ __launch_bounds__(GEMM_THREADS) void mlp_gemm(GemmArgs g)
 {
     __shared__ __attribute__((aligned(16))) uint16_t As[2][BM * LDS_STRIDE];
@@ -135,34 +134,37 @@ __global__ __launch_bounds__(GEMM_THREADS) void mlp_gemm(GemmArgs g)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    uint4 ra = load_tile<AK>(g.A, g.lda, m0, kbeg, tid);
-    uint4 rb = load_tile<BKM>(g.B, g.ldb, n0, kbeg, tid);
-    store_tile<AK>(As[0], ra, tid);
-    store_tile<BKM>(Bs[0], rb, tid);
+    Tile ra = load_tile(g.A, g.lda, m0, kbeg, tid);
+    Tile rb = load_tile(g.B, g.ldb, n0, kbeg, tid);
+    store_tile(As[0], ra, tid);
+    store_tile(Bs[0], rb, tid);
     __syncthreads();
 
     const uint32_t r16 = lane & 15u, kg = (lane >> 4) * 8u;
     for (int32_t s = 0; s < nk; s++) {
         const int cur = s & 1;
         if (s + 1 < nk) {   /* next k step in flight while this one multiplies */
-            ra = load_tile<AK>(g.A, g.lda, m0, kbeg + (s + 1) * BK, tid);
-            rb = load_tile<BKM>(g.B, g.ldb, n0, kbeg + (s + 1) * BK, tid);
+            ra = load_tile(g.A, g.lda, m0, kbeg + (s + 1) * BK, tid);
+            rb = load_tile(g.B, g.ldb, n0, kbeg + (s + 1) * BK, tid);
         }
-        bf16x8 a[2], b[2];
 #pragma unroll
-        for (int i = 0; i < 2; i++)
-            a[i] = *(const bf16x8 *)(As[cur] + (wm * 32 + i * 16 + r16) * LDS_STRIDE + kg);
+        for (int kk = 0; kk < BK; kk += 32) {
+            bf16x8 a[2], b[2];
 #pragma unroll
-        for (int j = 0; j < 2; j++)
-            b[j] = *(const bf16x8 *)(Bs[cur] + (wn * 32 + j * 16 + r16) * LDS_STRIDE + kg);
-#pragma unroll
-        for (int i = 0; i < 2; i++)
+            for (int i = 0; i < 2; i++)
+                a[i] = *(const bf16x8 *)(As[cur] + (wm * 32 + i * 16 + r16) * LDS_STRIDE + kk + kg);
 #pragma unroll
             for (int j = 0; j < 2; j++)
-                acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+                b[j] = *(const bf16x8 *)(Bs[cur] + (wn * 32 + j * 16 + r16) * LDS_STRIDE + kk + kg);
+#pragma unroll
+            for (int i = 0; i < 2; i++)
+#pragma unroll
+                for (int j = 0; j < 2; j++)
+                    acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
+        }
         if (s + 1 < nk) {
-            store_tile<AK>(As[cur ^ 1], ra, tid);
-            store_tile<BKM>(Bs[cur ^ 1], rb, tid);
+            store_tile(As[cur ^ 1], ra, tid);
+            store_tile(Bs[cur ^ 1], rb, tid);
         }
         __syncthreads();
     }
@@ -175,23 +177,32 @@ __global__ __launch_bounds__(GEMM_THREADS) void mlp_gemm(GemmArgs g)
 #pragma unroll
         for (int j = 0; j < 2; j++) {
             const int32_t col = n0 + wn * 32 + j * 16 + r16;
+            const int32_t row0 = m0 + wm * 32 + i * 16 + rq;
+            uint16_t h[4];
 #pragma unroll
             for (int r = 0; r < 4; r++) {
-                const int32_t row = m0 + wm * 32 + i * 16 + rq + r;
+                const int32_t row = row0 + r;
                 const float v = acc[i][j][r];
                 if (EPI == EPI_RELU_BF16) {
                     const float z = v + g.bias[col];
-                    g.out_bf[(int64_t)row * g.ldo + col] = f2bf(z > 0.f ? z : 0.f);
+                    h[r] = f2bf(z > 0.f ? z : 0.f);
+                    g.out_bf[(int64_t)row * g.ldo + col] = h[r];
                 } else if (EPI == EPI_LINEAR_F32) {
                     g.out_f[(int64_t)row * g.ldo + col] = v + g.bias[col];
                 } else if (EPI == EPI_DRELU_BF16) {
-                    const uint16_t h = g.mask[(int64_t)row * g.ldo + col];
-                    const uint16_t dz = (h != 0 && !(h & 0x8000u)) ? f2bf(v) : (uint16_t)0;
-                    g.out_bf[(int64_t)row * g.ldo + col] = dz;
-                    csum[j] += bf2f(dz);
+                    const uint16_t m = g.mask[(int64_t)row * g.ldo + col];
+                    h[r] = (m != 0 && !(m & 0x8000u)) ? f2bf(v) : (uint16_t)0;
+                    g.out_bf[(int64_t)row * g.ldo + col] = h[r];
+                    csum[j] += bf2f(h[r]);
                 } else {
                     g.out_f[((int64_t)blockIdx.z * g.M + row) * g.ldo + col] = v;
                 }
+            }
+            if ((EPI == EPI_RELU_BF16 || EPI == EPI_DRELU_BF16) && g.out_t) {
+                /* 4 consecutive rows of one column: one 8-byte store */
+                const uint2 q = {(uint32_t)h[0] | ((uint32_t)h[1] << 16),
+                                 (uint32_t)h[2] | ((uint32_t)h[3] << 16)};
+                *(uint2 *)(g.out_t + (int64_t)col * g.ldt + row0) = q;
             }
         }
     }
@@ -209,47 +220,69 @@ __global__ __launch_bounds__(GEMM_THREADS) void mlp_gemm(GemmArgs g)
     }
 }
 
-/* Loss of a minibatch: rows [64 * blockIdx.x, + 64), one thread per output
- * column.  dZ = bf16(scale * (out - y)) (0 in padded columns); partial sums of
- * (out - y)^2 and of the bf16 dZ per column. */
+/* Loss of a minibatch: workgroup (x, y) takes rows [64 x, + 64) and output
+ * columns [64 y, + 64); thread (c, q) = (tid % 64, tid / 64) takes column
+ * 64 y + c over rows 64 x + 16 q .. + 15.  dZ = bf16(scale * (out - y)) (0 in
+ * padded columns), written row-major and transposed; partial sums of
+ * (out - y)^2 per workgroup and of the bf16 dZ per column and 64 rows, each in
+ * a fixed order. */
 __global__ __launch_bounds__(256) void mlp_loss(const float *out, const float *y, uint16_t *dz,
-                                                float *bpart, float *lpart, int32_t npad,
-                                                int32_t nvalid, float scale)
+                                                uint16_t *dz_t, int64_t ldt, float *bpart,
+                                                float *lpart, int32_t npad, int32_t nvalid,
+                                                float scale)
 {
+    __shared__ float cs[4][64];
     __shared__ float red[256];
-    const int32_t r0 = blockIdx.x * ROWS_PER_PART;
-    float lsum = 0.f;
-    for (int32_t c = threadIdx.x; c < npad; c += blockDim.x) {
-        float cs = 0.f;
-        for (int32_t r = r0; r < r0 + ROWS_PER_PART; r++) {
-            uint16_t h = 0;
-            if (c < nvalid) {
-                const float diff = out[(int64_t)r * npad + c] - y[(int64_t)r * nvalid + c];
-                lsum += diff * diff;
-                h = f2bf(scale * diff);
-            }
-            dz[(int64_t)r * npad + c] = h;
-            cs += bf2f(h);
+    const uint32_t tid = threadIdx.x, c = tid & 63u, q = tid >> 6;
+    const int32_t col = blockIdx.y * 64 + c;
+    const int32_t r0 = blockIdx.x * ROWS_PER_PART + q * 16;
+    float lsum = 0.f, csum = 0.f;
+    uint16_t h[16];
+#pragma unroll
+    for (int r = 0; r < 16; r++) {
+        const int32_t row = r0 + r;
+        h[r] = 0;
+        if (col < nvalid) {
+            const float diff = out[(int64_t)row * npad + col] - y[(int64_t)row * nvalid + col];
+            lsum += diff * diff;
+            h[r] = f2bf(scale * diff);
         }
-        bpart[(int64_t)blockIdx.x * npad + c] = cs;
+        dz[(int64_t)row * npad + col] = h[r];
+        csum += bf2f(h[r]);
     }
-    red[threadIdx.x] = lsum;
+    uint4 *t = (uint4 *)(dz_t + (int64_t)col * ldt + r0);
+#pragma unroll
+    for (int k = 0; k < 2; k++) {
+        uint32_t w[4];
+#pragma unroll
+        for (int j = 0; j < 4; j++)
+            w[j] = (uint32_t)h[8 * k + 2 * j] | ((uint32_t)h[8 * k + 2 * j + 1] << 16);
+        t[k] = uint4{w[0], w[1], w[2], w[3]};
+    }
+    cs[q][c] = csum;
+    red[tid] = lsum;
     __syncthreads();
+    if (q == 0) bpart[(int64_t)blockIdx.x * npad + col] = ((cs[0][c] + cs[1][c]) + cs[2][c]) + cs[3][c];
     for (int s = 128; s > 0; s >>= 1) {
-        if ((int)threadIdx.x < s) red[threadIdx.x] += red[threadIdx.x + s];
+        if ((int)tid < s) red[tid] += red[tid + s];
         __syncthreads();
     }
-    if (threadIdx.x == 0) lpart[blockIdx.x] = red[0];
+    if (tid == 0) lpart[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
 }
 
 /* Minibatch gather: X rows (bf16, 64 wide) and Y rows of the resident set. */
 __global__ __launch_bounds__(256) void mlp_gather(const uint16_t *xall, const float *yall,
-                                                  const int32_t *rows, uint16_t *x, float *y,
-                                                  int32_t kin, int32_t nvalid)
+                                                  const int32_t *rows, uint16_t *x, uint16_t *x_t,
+                                                  float *y, int32_t batch, int32_t kin,
+                                                  int32_t nvalid)
 {
     const int32_t b = blockIdx.x;
     const int64_t src = rows[b];
-    for (int32_t c = threadIdx.x; c < kin; c += blockDim.x) x[(int64_t)b * kin + c] = xall[src * kin + c];
+    for (int32_t c = threadIdx.x; c < kin; c += blockDim.x) {
+        const uint16_t v = xall[src * kin + c];
+        x[(int64_t)b * kin + c] = v;
+        x_t[(int64_t)c * batch + b] = v;
+    }
     for (int32_t c = threadIdx.x; c < nvalid; c += blockDim.x)
         y[(int64_t)b * nvalid + c] = yall[src * nvalid + c];
 }
@@ -290,6 +323,7 @@ __global__ __launch_bounds__(256) void mlp_adam(AdamArgs a)
     const AdamSeg &sg = a.seg[sidx];
     const int64_t j = i - sg.off;
     float gsum = 0.f;
+#pragma unroll 8
     for (int32_t k = 0; k < sg.nparts; k++) gsum += sg.parts[(int64_t)k * sg.count + j];
     if (a.grad) a.grad[i] = gsum;
     if (!a.update) return;
@@ -304,21 +338,49 @@ __global__ __launch_bounds__(256) void mlp_adam(AdamArgs a)
 }
 
 /* Input conversion: f32 [n][in_dim] -> bf16 [n][kin] (zero padded). */
-__global__ __launch_bounds__(256) void mlp_convert_x(const float *x, uint16_t *xb, int64_t n,
-                                                     int32_t in_dim, int32_t kin)
+__global__ __launch_bounds__(256) void mlp_convert_x(const float *x, uint16_t *xb, uint16_t *xb_t,
+                                                     int64_t n, int32_t in_dim, int32_t kin)
 {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n * kin) return;
     const int64_t r = i / kin;
     const int32_t c = (int32_t)(i % kin);
-    xb[i] = c < in_dim ? f2bf(x[r * in_dim + c]) : (uint16_t)0;
+    const uint16_t v = c < in_dim ? f2bf(x[r * in_dim + c]) : (uint16_t)0;
+    xb[i] = v;
+    if (xb_t) xb_t[(int64_t)c * n + r] = v;
 }
 
-template <bool AK, bool BKM, int EPI>
+struct TransArgs {
+    int64_t off[MAX_LAYERS];    /* weight matrix l at off[l] in both copies */
+    int32_t n[MAX_LAYERS], k[MAX_LAYERS];
+    int32_t first[MAX_LAYERS + 1];   /* first 64 x 64 tile of matrix l */
+    int32_t nlin;
+    const uint16_t *w;           /* W_l [n][k] */
+    uint16_t *wt;                /* W_l^T [k][n] */
+};
+
+/* bf16 weight copies transposed through LDS, one 64 x 64 tile per workgroup. */
+__global__ __launch_bounds__(256) void mlp_transpose_w(TransArgs a)
+{
+    __shared__ uint16_t t[64][66];
+    int l = 0;
+    while (l + 1 < a.nlin && (int32_t)blockIdx.x >= a.first[l + 1]) l++;
+    const int32_t tile = blockIdx.x - a.first[l];
+    const int32_t tk = a.k[l] / 64;
+    const int32_t r0 = (tile / tk) * 64, c0 = (tile % tk) * 64;
+    const uint16_t *w = a.w + a.off[l];
+    uint16_t *wt = a.wt + a.off[l];
+    const uint32_t x = threadIdx.x & 63u, y = threadIdx.x >> 6;
+    for (uint32_t r = y; r < 64; r += 4) t[r][x] = w[(int64_t)(r0 + r) * a.k[l] + c0 + x];
+    __syncthreads();
+    for (uint32_t c = y; c < 64; c += 4) wt[(int64_t)(c0 + c) * a.n[l] + r0 + x] = t[x][c];
+}
+
+template <int EPI>
 hipError_t gemm(const GemmArgs &g, int32_t splits, hipStream_t st)
 {
     const dim3 grid(g.N / BN, g.M / BM, splits);
-    hipLaunchKernelGGL((mlp_gemm<AK, BKM, EPI>), grid, dim3(GEMM_THREADS), 0, st, g);
+    hipLaunchKernelGGL((mlp_gemm<EPI>), grid, dim3(GEMM_THREADS), 0, st, g);
     return hipGetLastError();
 }
 
@@ -337,9 +399,12 @@ struct bsim_mlp {
     std::vector<void *> allocs;
     float *p = nullptr, *m = nullptr, *v = nullptr, *grad = nullptr;
     uint16_t *pbf = nullptr;
+    uint16_t *pbf_t = nullptr;        /* weights transposed (W_l^T [k][n] at woff[l]) */
     uint16_t *act[MAX_LAYERS + 1]{};  /* act[0] input, act[l] hidden l (bf16) */
     float *out = nullptr, *ybat = nullptr;
     uint16_t *dz[2]{};                /* activation gradients (ping-pong) */
+    uint16_t *act_t[MAX_LAYERS]{};    /* act[l] transposed, [kdim[l]][batch] */
+    uint16_t *dz_t[2]{};              /* dz transposed, [width][batch] */
     float *wpart[MAX_LAYERS]{}, *bpart[MAX_LAYERS]{};
     float *lpart = nullptr, *loss = nullptr;
     uint16_t *xall = nullptr;
@@ -383,8 +448,9 @@ double hash_uniform(uint64_t seed, uint64_t i)
     return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
-/* forward pass on act[0] (batch rows); fp32 output in m->out */
-int forward(bsim_mlp *m, int32_t batch)
+/* forward pass on act[0] (batch rows); fp32 output in m->out; with
+ * transposed, the hidden activations are also written to act_t */
+int forward(bsim_mlp *m, int32_t batch, bool transposed)
 {
     for (int l = 0; l < m->nlin; l++) {
         GemmArgs g{};
@@ -400,10 +466,12 @@ int forward(bsim_mlp *m, int32_t batch)
         hipError_t e;
         if (l + 1 < m->nlin) {
             g.out_bf = m->act[l + 1];
-            e = gemm<true, true, EPI_RELU_BF16>(g, 1, m->stream);
+            g.out_t = transposed ? m->act_t[l + 1] : nullptr;
+            g.ldt = batch;
+            e = gemm<EPI_RELU_BF16>(g, 1, m->stream);
         } else {
             g.out_f = m->out;
-            e = gemm<true, true, EPI_LINEAR_F32>(g, 1, m->stream);
+            e = gemm<EPI_LINEAR_F32>(g, 1, m->stream);
         }
         if (e != hipSuccess) return hip_check(e, "forward launch");
     }
@@ -417,32 +485,35 @@ int backward(bsim_mlp *m, int32_t batch)
     const int L = m->nlin;
     const int32_t nout = m->kdim[L];
     const float scale = 2.0f / ((float)batch * (float)m->cfg.out_dim);
-    hipLaunchKernelGGL(mlp_loss, dim3(batch / ROWS_PER_PART), dim3(256), 0, m->stream, m->out,
-                       m->ybat, m->dz[0], m->bpart[L - 1], m->lpart, nout, m->cfg.out_dim, scale);
+    hipLaunchKernelGGL(mlp_loss, dim3(batch / ROWS_PER_PART, nout / 64), dim3(256), 0, m->stream,
+                       m->out, m->ybat, m->dz[0], m->dz_t[0], (int64_t)batch, m->bpart[L - 1],
+                       m->lpart, nout, m->cfg.out_dim, scale);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_check(e, "loss launch");
     int cur = 0;
     for (int l = L - 1; l >= 0; l--) {
-        /* dW_l [kdim[l+1]][kdim[l]] = dZ^T H_{l}: A(m = out unit, k = row), B(n = in unit, k = row) */
+        /* dW_l [kdim[l+1]][kdim[l]] = dZ^T H_{l}: A(m = out unit, k = row) = dz_t[m][k],
+         * B(n = in unit, k = row) = act_t[l][n][k] */
         GemmArgs gw{};
-        gw.A = m->dz[cur];
-        gw.lda = m->kdim[l + 1];
-        gw.B = m->act[l];
-        gw.ldb = m->kdim[l];
+        gw.A = m->dz_t[cur];
+        gw.lda = batch;
+        gw.B = m->act_t[l];
+        gw.ldb = batch;
         gw.M = m->kdim[l + 1];
         gw.N = m->kdim[l];
         gw.K = batch / KSPLIT;
         gw.out_f = m->wpart[l];
         gw.ldo = m->kdim[l];
-        e = gemm<false, false, EPI_PARTIAL_F32>(gw, KSPLIT, m->stream);
+        e = gemm<EPI_PARTIAL_F32>(gw, KSPLIT, m->stream);
         if (e != hipSuccess) return hip_check(e, "weight-gradient launch");
         if (l == 0) break;
-        /* dZ_{l-1} [batch][kdim[l]] = (dZ W_l) where H > 0: B(n = in unit, k = out unit) = W[k][n] */
+        /* dZ_{l-1} [batch][kdim[l]] = (dZ W_l) where H > 0:
+         * B(n = in unit, k = out unit) = W[k][n] = W^T[n][k] */
         GemmArgs gx{};
         gx.A = m->dz[cur];
         gx.lda = m->kdim[l + 1];
-        gx.B = m->pbf + m->woff[l];
-        gx.ldb = m->kdim[l];
+        gx.B = m->pbf_t + m->woff[l];
+        gx.ldb = m->kdim[l + 1];
         gx.M = batch;
         gx.N = m->kdim[l];
         gx.K = m->kdim[l + 1];
@@ -450,7 +521,9 @@ int backward(bsim_mlp *m, int32_t batch)
         gx.ldo = m->kdim[l];
         gx.mask = m->act[l];
         gx.bpart = m->bpart[l - 1];
-        e = gemm<true, false, EPI_DRELU_BF16>(gx, 1, m->stream);
+        gx.out_t = m->dz_t[cur ^ 1];
+        gx.ldt = batch;
+        e = gemm<EPI_DRELU_BF16>(gx, 1, m->stream);
         if (e != hipSuccess) return hip_check(e, "activation-gradient launch");
         cur ^= 1;
     }
@@ -479,7 +552,7 @@ int adam(bsim_mlp *m, int32_t batch, bool update, float lr, float *grad_out, flo
     a.grad = grad_out;
     a.pbf = m->pbf;
     a.lpart = m->lpart;
-    a.nlpart = batch / ROWS_PER_PART;
+    a.nlpart = batch / ROWS_PER_PART * (m->kdim[m->nlin] / 64);
     a.lscale = 1.0f / ((float)batch * (float)m->cfg.out_dim);
     a.loss = loss_dev;
     const int64_t blocks = (m->nparam + 255) / 256;
@@ -488,6 +561,23 @@ int adam(bsim_mlp *m, int32_t batch, bool update, float lr, float *grad_out, flo
     if (e != hipSuccess) return hip_check(e, "adam launch");
     if (update) m->step++;
     return BSIM_OK;
+}
+
+int transpose_weights(bsim_mlp *m)
+{
+    TransArgs a{};
+    a.nlin = m->nlin;
+    a.first[0] = 0;
+    for (int l = 0; l < m->nlin; l++) {
+        a.off[l] = m->woff[l];
+        a.n[l] = m->kdim[l + 1];
+        a.k[l] = m->kdim[l];
+        a.first[l + 1] = a.first[l] + (a.n[l] / 64) * (a.k[l] / 64);
+    }
+    a.w = m->pbf;
+    a.wt = m->pbf_t;
+    hipLaunchKernelGGL(mlp_transpose_w, dim3(a.first[m->nlin]), dim3(256), 0, m->stream, a);
+    return hip_check(hipGetLastError(), "transpose launch");
 }
 
 int check_batch(const bsim_mlp *m, int32_t batch)
@@ -551,6 +641,7 @@ bsim_mlp *bsim_mlp_create(const bsim_mlp_config *cfg)
     if (!rc) rc = dalloc(m, &m->v, m->nparam);
     if (!rc) rc = dalloc(m, &m->grad, m->nparam);
     if (!rc) rc = dalloc(m, &m->pbf, m->nparam);
+    if (!rc) rc = dalloc(m, &m->pbf_t, m->nparam);
     for (int l = 0; l < m->nlin && !rc; l++) rc = dalloc(m, &m->act[l], B * m->kdim[l]);
     if (!rc) rc = dalloc(m, &m->out, B * m->kdim[m->nlin]);
     if (!rc) rc = dalloc(m, &m->ybat, B * c.out_dim);
@@ -558,11 +649,14 @@ bsim_mlp *bsim_mlp_create(const bsim_mlp_config *cfg)
     for (int l = 0; l <= m->nlin; l++) wmax = std::max(wmax, m->kdim[l]);
     if (!rc) rc = dalloc(m, &m->dz[0], B * wmax);
     if (!rc) rc = dalloc(m, &m->dz[1], B * wmax);
+    if (!rc) rc = dalloc(m, &m->dz_t[0], B * wmax);
+    if (!rc) rc = dalloc(m, &m->dz_t[1], B * wmax);
+    for (int l = 0; l < m->nlin && !rc; l++) rc = dalloc(m, &m->act_t[l], B * m->kdim[l]);
     for (int l = 0; l < m->nlin && !rc; l++) {
         rc = dalloc(m, &m->wpart[l], (int64_t)KSPLIT * m->kdim[l + 1] * m->kdim[l]);
         if (!rc) rc = dalloc(m, &m->bpart[l], B / ROWS_PER_PART * m->kdim[l + 1]);
     }
-    if (!rc) rc = dalloc(m, &m->lpart, B / ROWS_PER_PART);
+    if (!rc) rc = dalloc(m, &m->lpart, B / ROWS_PER_PART * (m->kdim[m->nlin] / 64));
     if (!rc) rc = dalloc(m, &m->loss, 1);
     if (!rc) rc = dalloc(m, &m->xstage, B * c.in_dim);
     if (rc != BSIM_OK) return bad(0);
@@ -628,6 +722,8 @@ int bsim_mlp_set_params(bsim_mlp *m, const float *in)
     for (int64_t i = 0; i < m->nparam; i++) pb[i] = f2bf(p[i]);
     int rc = hip_check(hipMemcpy(m->p, p.data(), m->nparam * 4, hipMemcpyHostToDevice), "hipMemcpy");
     if (!rc) rc = hip_check(hipMemcpy(m->pbf, pb.data(), m->nparam * 2, hipMemcpyHostToDevice), "hipMemcpy");
+    if (!rc) rc = transpose_weights(m);
+    if (!rc) rc = hip_check(hipStreamSynchronize(m->stream), "sync");
     if (!rc) rc = hip_check(hipMemset(m->m, 0, m->nparam * 4), "hipMemset");
     if (!rc) rc = hip_check(hipMemset(m->v, 0, m->nparam * 4), "hipMemset");
     m->step = 0;
@@ -679,12 +775,13 @@ int bsim_mlp_train(bsim_mlp *m, const int32_t *rows, int32_t steps, int32_t batc
                             "hipMemcpy");
     for (int32_t s = 0; s < steps && !rc; s++) {
         hipLaunchKernelGGL(mlp_gather, dim3(batch), dim3(256), 0, m->stream, m->xall, m->yall,
-                           rdev + (int64_t)s * batch, m->act[0], m->ybat, m->kdim[0],
-                           m->cfg.out_dim);
+                           rdev + (int64_t)s * batch, m->act[0], m->act_t[0], m->ybat, batch,
+                           m->kdim[0], m->cfg.out_dim);
         rc = hip_check(hipGetLastError(), "gather launch");
-        if (!rc) rc = forward(m, batch);
+        if (!rc) rc = forward(m, batch, true);
         if (!rc) rc = backward(m, batch);
         if (!rc) rc = adam(m, batch, true, lr, nullptr, ldev + s);
+        if (!rc) rc = transpose_weights(m);
     }
     const int src = hip_check(hipStreamSynchronize(m->stream), "sync");
     if (!rc) rc = src;
@@ -711,9 +808,9 @@ int bsim_mlp_gradients(bsim_mlp *m, const float *x, const float *y, int32_t batc
     if (rc) return rc;
     const int64_t tot = (int64_t)batch * kin;
     hipLaunchKernelGGL(mlp_convert_x, dim3((uint32_t)((tot + 255) / 256)), dim3(256), 0, m->stream,
-                       m->xstage, m->act[0], (int64_t)batch, din, kin);
+                       m->xstage, m->act[0], m->act_t[0], (int64_t)batch, din, kin);
     rc = hip_check(hipGetLastError(), "convert launch");
-    if (!rc) rc = forward(m, batch);
+    if (!rc) rc = forward(m, batch, true);
     if (!rc) rc = backward(m, batch);
     if (!rc) rc = adam(m, batch, false, 0.f, m->grad, m->loss);
     if (!rc) rc = hip_check(hipStreamSynchronize(m->stream), "sync");
@@ -739,9 +836,10 @@ int bsim_mlp_predict(bsim_mlp *m, const float *x, int64_t n, float *y)
         if (rc) break;
         const int64_t tot = (int64_t)batch * kin;
         hipLaunchKernelGGL(mlp_convert_x, dim3((uint32_t)((tot + 255) / 256)), dim3(256), 0,
-                           m->stream, m->xstage, m->act[0], (int64_t)batch, din, kin);
+                           m->stream, m->xstage, m->act[0], (uint16_t *)nullptr, (int64_t)batch,
+                           din, kin);
         rc = hip_check(hipGetLastError(), "convert launch");
-        if (!rc) rc = forward(m, batch);
+        if (!rc) rc = forward(m, batch, false);
         if (rc) break;
         tmp.resize((size_t)rows * nout);
         rc = hip_check(hipMemcpyAsync(tmp.data(), m->out, (size_t)rows * nout * 4,

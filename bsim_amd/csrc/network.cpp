@@ -7,6 +7,7 @@
 #include "network.h"
 
 #include <math.h>
+#include <omp.h>
 
 #include <algorithm>
 #include <map>
@@ -79,6 +80,39 @@ int32_t default_tile(int32_t n, int32_t cus)
     return BSIM_TILE_ALIGN;
 }
 
+/* Weight and delay steps of synapse k (StaticSynapse::reset); 0, or a
+ * negative code with the message in *err when err is not null. */
+static int reset_synapse(const HostNet &net, int64_t k, int32_t N, float dt, float *w_out,
+                         int32_t *steps, std::string *err)
+{
+    const int32_t a = net.pre[k], b = net.post[k];
+    if (a < 0 || a >= N || b < 0 || b >= N) {
+        if (err) *err = "synapse " + std::to_string(k) + " has an endpoint outside the network";
+        return -2;
+    }
+    const float tau_in = net.tau.empty() ? 0.0f : net.tau[k];
+    float tau;
+    if (fabsf(tau_in) > kZero) {
+        tau = tau_in;
+    } else {
+        const float *pd = &net.lif[(size_t)b * kLifParams];
+        tau = net.stype[k] ? pd[11] : pd[10];
+    }
+    if (!synapse_reset(net.w[k], net.d[k], tau, dt, w_out, steps)) {
+        if (err)
+            *err = "synapse " + std::to_string(k) +
+                   ": delay is not a whole number of steps and its tau is zero";
+        return -3;
+    }
+    if (*steps < 1 || *steps > BSIM_MAX_DELAY) {
+        if (err)
+            *err = "synapse " + std::to_string(k) + ": delay of " + std::to_string(*steps) +
+                   " steps is outside 1.." + std::to_string(BSIM_MAX_DELAY);
+        return -4;
+    }
+    return 0;
+}
+
 int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::string &err)
 {
     const int32_t N = net.n();
@@ -131,70 +165,62 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
     }
     L.arr_t = net.arr_t;
 
-    /* synapse reset: weight and delay steps */
+    /* synapse reset: weight and delay steps (parallel; on failure the first
+     * failing synapse is re-run serially for its message) */
     std::vector<float> w(S);
     std::vector<uint8_t> ds(S);
     int32_t dmax = 1;
+    int64_t first_bad = S;
+#pragma omp parallel for schedule(static) reduction(max : dmax) reduction(min : first_bad)
     for (int64_t k = 0; k < S; k++) {
-        const int32_t a = net.pre[k], b = net.post[k];
-        if (a < 0 || a >= N || b < 0 || b >= N) {
-            err = "synapse " + std::to_string(k) + " has an endpoint outside the network";
-            return -2;
-        }
-        float tau_in = net.tau.empty() ? 0.0f : net.tau[k];
-        float tau;
-        if (fabsf(tau_in) > kZero) {
-            tau = tau_in;
-        } else {
-            const float *pd = &net.lif[(size_t)b * kLifParams];
-            tau = net.stype[k] ? pd[11] : pd[10];
-        }
         int32_t steps;
-        if (!synapse_reset(net.w[k], net.d[k], tau, dt, &w[k], &steps)) {
-            err = "synapse " + std::to_string(k) +
-                  ": delay is not a whole number of steps and its tau is zero";
-            return -3;
-        }
-        if (steps < 1 || steps > BSIM_MAX_DELAY) {
-            err = "synapse " + std::to_string(k) + ": delay of " + std::to_string(steps) +
-                  " steps is outside 1.." + std::to_string(BSIM_MAX_DELAY);
-            return -4;
+        if (reset_synapse(net, k, N, dt, &w[k], &steps, nullptr) != 0) {
+            if (k < first_bad) first_bad = k;
+            continue;
         }
         ds[k] = (uint8_t)steps;
         if (steps > dmax) dmax = steps;
     }
+    if (first_bad < S) {
+        float wk;
+        int32_t st;
+        return reset_synapse(net, first_bad, N, dt, &wk, &st, &err);
+    }
     L.dmax = dmax;
 
-    /* Ordering property: the GPU sums a target's inputs in (pre, delay,
-     * creation) order; the reference sums them in creation order
-     * (Network.cpp:324-327).  They agree when each target's incoming
-     * synapses, taken in creation order, have non-decreasing (pre, delay). */
-    {
-        std::vector<int64_t> last(N, -1);
-        for (int64_t k = 0; k < S; k++) {
-            const int64_t key = ((int64_t)net.pre[k] << 8) | ds[k];
-            int64_t &l = last[net.post[k]];
-            if (key < l) {
-                err = "synapse " + std::to_string(k) + " (" + std::to_string(net.pre[k]) + " -> " +
-                      std::to_string(net.post[k]) +
-                      ") was created after a synapse into the same target from a later "
-                      "(pre, delay); the GPU step requires each target's inputs to be created "
-                      "in ascending (pre, delay) order";
-                return -5;
-            }
-            l = key;
-        }
-    }
-
-    /* bucket synapses by target tile, keeping creation order */
+    /* bucket synapses by target tile, keeping creation order: a parallel
+     * stable counting sort (per-thread counts, prefix over (tile, thread)) */
     const int32_t NT = L.ntiles;
     L.tile_syn.assign((size_t)NT + 1, 0);
-    for (int64_t k = 0; k < S; k++) L.tile_syn[net.post[k] / tile + 1]++;
-    for (int32_t t = 0; t < NT; t++) L.tile_syn[t + 1] += L.tile_syn[t];
     std::vector<int64_t> order(S);
     {
-        std::vector<int64_t> fill(L.tile_syn.begin(), L.tile_syn.end() - 1);
-        for (int64_t k = 0; k < S; k++) order[fill[net.post[k] / tile]++] = k;
+        const int nmax = omp_get_max_threads();
+        std::vector<int64_t> cnt((size_t)nmax * NT, 0);
+        int nth = 1;
+#pragma omp parallel
+        {
+            const int id = omp_get_thread_num(), T = omp_get_num_threads();
+#pragma omp single
+            nth = T;
+            const int64_t k0 = S * id / T, k1 = S * (id + 1) / T;
+            int64_t *c = cnt.data() + (size_t)id * NT;
+            for (int64_t k = k0; k < k1; k++) c[net.post[k] / tile]++;
+#pragma omp barrier
+#pragma omp single
+            {
+                int64_t run = 0;
+                for (int32_t t = 0; t < NT; t++) {
+                    L.tile_syn[t] = run;
+                    for (int j = 0; j < nth; j++) {
+                        const int64_t x = cnt[(size_t)j * NT + t];
+                        cnt[(size_t)j * NT + t] = run;
+                        run += x;
+                    }
+                }
+                L.tile_syn[NT] = run;
+            }
+            for (int64_t k = k0; k < k1; k++) order[c[net.post[k] / tile]++] = k;
+        }
     }
     for (int32_t t = 0; t < NT; t++) {
         if (L.tile_syn[t + 1] - L.tile_syn[t] >= ((int64_t)1 << 32)) {
@@ -211,17 +237,29 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
     const int32_t NW = BSIM_NWAVE;
     const int32_t qw = tile / NW;   /* targets per part */
     int fail = 0;
+    int64_t first_unordered = S;
 
-#pragma omp parallel for schedule(dynamic, 1)
+    /* Ordering property: the GPU sums a target's inputs in (pre, delay,
+     * creation) order; the reference sums them in creation order
+     * (Network.cpp:324-327).  They agree when each target's incoming
+     * synapses, taken in creation order, have non-decreasing (pre, delay).
+     * Checked per tile below (a tile's bucket keeps creation order); the
+     * first offending synapse over all tiles is reported. */
+#pragma omp parallel for schedule(dynamic, 1) reduction(min : first_unordered)
     for (int32_t t = 0; t < NT; t++) {
         const int64_t lo = L.tile_syn[t], hi = L.tile_syn[t + 1], m = hi - lo;
         /* row key: pre (31 bits) | delay (6 bits) | part (2 bits); ties
          * keep the creation rank */
         struct Ent { uint64_t key; uint32_t rank; };
         std::vector<Ent> ent(m);
+        std::vector<int64_t> last(tile, -1);
         for (int64_t e = 0; e < m; e++) {
             const int64_t k = order[lo + e];
-            const uint32_t q = (uint32_t)((net.post[k] - t * tile) / qw);
+            const int32_t lt = net.post[k] - t * tile;
+            const int64_t okey = ((int64_t)net.pre[k] << 8) | ds[k];
+            if (okey < last[lt] && k < first_unordered) first_unordered = k;
+            last[lt] = okey;
+            const uint32_t q = (uint32_t)(lt / qw);
             ent[e].key = ((uint64_t)(uint32_t)net.pre[k] << 8) | ((uint64_t)ds[k] << 2) | q;
             ent[e].rank = (uint32_t)e;
         }
@@ -272,6 +310,15 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
             trng[t].push_back(r);
             e = f;
         }
+    }
+    if (first_unordered < S) {
+        const int64_t k = first_unordered;
+        err = "synapse " + std::to_string(k) + " (" + std::to_string(net.pre[k]) + " -> " +
+              std::to_string(net.post[k]) +
+              ") was created after a synapse into the same target from a later "
+              "(pre, delay); the GPU step requires each target's inputs to be created "
+              "in ascending (pre, delay) order";
+        return -5;
     }
     if (fail) {
         err = "internal error: CSR row walk mismatch";

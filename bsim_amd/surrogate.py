@@ -259,10 +259,18 @@ def dataset(points, rates, size=sweep.C3["size"]):
     return X, Y, idx
 
 
-def fit(mlp, X, Y, epochs, batch=4096, lr=1e-3, seed=0, lr_final=None):
+def rmse_hz(mlp, X, Y):
+    """Rate RMSE (Hz) of the model on (X, Y)."""
+    d = (mlp.predict(X) - Y) * RATE_SCALE
+    return float(np.sqrt(np.mean(d.astype(np.float64) ** 2)))
+
+
+def fit(mlp, X, Y, epochs, batch=4096, lr=1e-3, seed=0, lr_final=None, val=None, val_every=50):
     """Trains on (X, Y) for the given epochs of shuffled minibatches (the last
-    partial batch of an epoch is dropped); lr decays geometrically to lr_final.
-    Returns (per-step losses, seconds)."""
+    partial batch of an epoch is dropped); lr decays geometrically to
+    lr_final.  With val = (Xv, Yv), the rate RMSE on it is checked every
+    val_every epochs and the best parameters are kept (early stopping).
+    Returns (per-step losses, seconds of training, {"best_epoch", "val_rmse_hz"})."""
     mlp.set_data(X, Y)
     rng = np.random.default_rng(seed)
     n = X.shape[0]
@@ -271,13 +279,22 @@ def fit(mlp, X, Y, epochs, batch=4096, lr=1e-3, seed=0, lr_final=None):
         raise ValueError(f"need at least one batch of {batch} rows, have {n}")
     losses, secs = [], 0.0
     lr_final = lr if lr_final is None else lr_final
+    best = (np.inf, None, -1)
     for e in range(epochs):
         rows = rng.permutation(n)[:per * batch].reshape(per, batch).astype(np.int32)
         rate = lr * (lr_final / lr) ** (e / max(1, epochs - 1))
         t0 = time.perf_counter()
         losses.append(mlp.train(rows, batch, rate))
         secs += time.perf_counter() - t0
-    return np.concatenate(losses), secs
+        if val is not None and ((e + 1) % val_every == 0 or e + 1 == epochs):
+            r = rmse_hz(mlp, *val)
+            if r < best[0]:
+                best = (r, mlp.get_params(), e + 1)
+    info = {"best_epoch": epochs, "val_rmse_hz": None}
+    if val is not None:
+        mlp.set_params(best[1])
+        info = {"best_epoch": best[2], "val_rmse_hz": best[0]}
+    return np.concatenate(losses), secs, info
 
 
 # ---- screening -------------------------------------------------------------
