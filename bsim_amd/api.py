@@ -38,6 +38,7 @@ _SIGS = {
     "bsim_net_add_bernoulli": (C.c_int32, [P, C.c_int32, P, C.c_uint64]),
     "bsim_net_add_bernoulli_keyed": (C.c_int32, [P, C.c_int32, P, C.c_uint64, C.c_uint32]),
     "bsim_net_gen_chain": (C.c_int32, [P, P]),
+    "bsim_net_gen_chains": (C.c_int32, [P, P, C.c_int32, P]),
     "bsim_net_connect": (C.c_int64, [P, C.c_int64, P, P, P, P, P, P]),
     "bsim_net_num_neurons": (C.c_int32, [P]),
     "bsim_net_num_synapses": (C.c_int64, [P]),
@@ -141,6 +142,35 @@ class Network:
     @property
     def s(self):
         return lib().bsim_net_num_synapses(self._h)
+
+    @staticmethod
+    def chain_params(sizes, fan_in, w0, inh_frac, dmax, drive_rate_hz, n_drive, drive_fan_in,
+                     drive_w, seed, tau_refrac=2e-3, i_offset=(14.5, 16.5), tau_syn_E=5e-3,
+                     tau_syn_I=10e-3, v_thresh=15.0, dt=netspec.DT, out=None):
+        """A bsim_chain_params (filled into out if given)."""
+        cp = ChainParams() if out is None else out
+        cp.npop = len(sizes)
+        for i, x in enumerate(sizes):
+            cp.sizes[i] = int(x)
+        cp.fan_in, cp.w0, cp.inh_frac, cp.dmax = int(fan_in), w0, inh_frac, int(dmax)
+        cp.drive_rate_hz, cp.n_drive, cp.drive_fan_in = drive_rate_hz, int(n_drive), int(drive_fan_in)
+        cp.drive_w, cp.seed, cp.tau_refrac = drive_w, int(seed), tau_refrac
+        cp.ioff_lo, cp.ioff_hi = i_offset
+        cp.tau_syn_E, cp.tau_syn_I, cp.v_thresh, cp.dt = tau_syn_E, tau_syn_I, v_thresh, dt
+        return cp
+
+    def add_chains(self, kwargs_list):
+        """Appends several sweep networks at once (bsim_net_gen_chains: the
+        same networks as add_chain in order); returns their first ids."""
+        n = len(kwargs_list)
+        arr = (ChainParams * max(n, 1))()
+        for i, kw in enumerate(kwargs_list):
+            self.chain_params(out=arr[i], **kw)
+        first = np.zeros(max(n, 1), np.int32)
+        r = lib().bsim_net_gen_chains(self._h, arr, n, _p(first))
+        if r < 0:
+            _err(r, "bsim_net_gen_chains")
+        return first[:n].astype(np.int64)
 
     def add_chain(self, sizes, fan_in, w0, inh_frac, dmax, drive_rate_hz, n_drive, drive_fan_in,
                   drive_w, seed, tau_refrac=2e-3, i_offset=(14.5, 16.5), tau_syn_E=5e-3,

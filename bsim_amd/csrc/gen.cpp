@@ -198,142 +198,220 @@ extern "C" int64_t bsim_net_gen_standard(bsim_network *netp, const bsim_gen_para
     return h.s();
 }
 
-extern "C" int32_t bsim_net_gen_chain(bsim_network *netp, const bsim_chain_params *gp)
+namespace {
+
+/* One chain's place in the network and its projections. */
+struct ChainPlan {
+    int64_t first = 0;       /* first neuron id */
+    int64_t nn = 0;          /* neurons */
+    int64_t syn0 = 0;        /* first synapse index */
+    int64_t nsyn = 0;
+    bool drive = false;
+    std::vector<std::pair<int64_t, int32_t>> pops;   /* (global start, count) */
+    std::vector<Proj> proj;
+    std::vector<int64_t> proj_off;                   /* first synapse of each projection */
+};
+
+bool chain_valid(const bsim_chain_params &p)
 {
-    if (!netp || !gp || gp->npop < 1 || gp->npop > 8 || gp->fan_in < 0 || gp->dmax < 1 ||
-        gp->dmax > BSIM_MAX_DELAY || !(gp->dt > 0) || gp->n_drive < 0 || gp->drive_fan_in < 0) {
-        g_bsim_err = "bsim_net_gen_chain: bad parameters";
+    if (p.npop < 1 || p.npop > 8 || p.fan_in < 0 || p.dmax < 1 || p.dmax > BSIM_MAX_DELAY ||
+        !(p.dt > 0) || p.n_drive < 0 || p.drive_fan_in < 0)
+        return false;
+    for (int32_t i = 0; i < p.npop; i++)
+        if (p.sizes[i] < 1) return false;
+    return true;
+}
+
+/* Populations and projections of chain p placed at (first, syn0):
+ * drive -> 0 and i -> i+1, with a per-pre fan-out of
+ * round(fan_in * n_dst / n_src) distinct targets (clamped to [1, n_dst]). */
+ChainPlan plan_chain(const bsim_chain_params &p, int64_t first, int64_t syn0)
+{
+    ChainPlan c;
+    c.first = first;
+    c.syn0 = syn0;
+    c.nn = p.n_drive;
+    for (int32_t i = 0; i < p.npop; i++) c.nn += p.sizes[i];
+    c.drive = p.n_drive > 0 && p.drive_rate_hz > 0 && p.drive_fan_in > 0;
+    if (p.n_drive > 0) c.pops.push_back({first, p.n_drive});
+    int64_t at = first + p.n_drive;
+    for (int32_t i = 0; i < p.npop; i++) {
+        c.pops.push_back({at, p.sizes[i]});
+        at += p.sizes[i];
+    }
+    const int32_t pl = p.n_drive > 0 ? 1 : 0;   /* index of population 0 in pops */
+    if (c.drive) {
+        const double k = std::round((double)p.drive_fan_in * p.sizes[0] / (double)p.n_drive);
+        c.proj.push_back({0, pl, (int32_t)std::max(1.0, std::min(k, (double)p.sizes[0]))});
+    }
+    for (int32_t i = 0; i + 1 < p.npop && p.fan_in > 0; i++) {
+        const double k = std::round((double)p.fan_in * p.sizes[i + 1] / (double)p.sizes[i]);
+        c.proj.push_back({pl + i, pl + i + 1, (int32_t)std::max(1.0, std::min(k, (double)p.sizes[i + 1]))});
+    }
+    int64_t o = syn0;
+    for (const Proj &q : c.proj) {
+        c.proj_off.push_back(o);
+        o += (int64_t)c.pops[q.src].second * q.f;
+    }
+    c.nsyn = o - syn0;
+    return c;
+}
+
+/* Neuron rows of chain p (arrays already sized); draws keyed by chain-local
+ * indices. */
+void write_chain_neurons(bsim::HostNet &h, const bsim_chain_params &p, const ChainPlan &c,
+                         int64_t arr_end)
+{
+    const double pr = std::min(1.0, (double)p.drive_rate_hz * (double)p.dt);
+    const uint32_t thr = (uint32_t)std::min(4294967295.0, std::round(pr * 4294967296.0));
+    for (int64_t j = 0; j < c.nn; j++) {
+        const size_t g = (size_t)(c.first + j);
+        float *row = &h.lif[g * bsim::kLifParams];
+        h.arr_off[g + 1] = arr_end;
+        if (j < p.n_drive) {   /* drive sources: keys 0.. of the chain's own seed */
+            h.kind[g] = (int8_t)BSIM_KIND_BERNOULLI;
+            std::fill(row, row + bsim::kLifParams, 0.0f);
+            row[10] = 1e-3f;
+            row[11] = 1e-3f;
+            h.bern_p[g] = c.drive ? thr : 0u;
+            h.bern_seed[g] = p.seed;
+            h.bern_key[g] = (uint32_t)j;
+            continue;
+        }
+        const int64_t li = j - p.n_drive;   /* LIF index within the chain */
+        float ioff = 0.0f;
+        if (p.ioff_hi > p.ioff_lo)
+            ioff = (float)((double)p.ioff_lo + ((double)p.ioff_hi - (double)p.ioff_lo) *
+                                                   u01(h3(p.seed, TAG_IOFF << 32, (uint64_t)li)));
+        const float r[bsim::kLifParams] = {0.0f, 0.0f, 0.0f, 20e-3f, 20e-3f, p.tau_refrac,
+                                           p.tau_syn_E, p.tau_syn_I, p.v_thresh, ioff,
+                                           p.tau_syn_E, p.tau_syn_I};
+        std::copy(r, r + bsim::kLifParams, row);
+        h.kind[g] = (int8_t)BSIM_KIND_LIF;
+        h.bern_p[g] = 0u;
+        h.bern_seed[g] = 0ull;
+        h.bern_key[g] = 0u;
+    }
+}
+
+/* The f synapses of pre i of projection pi of chain p, pre-major, targets by
+ * stratified sampling; every draw keyed by (projection, local counter). */
+void write_pre_synapses(bsim::HostNet &h, const bsim_chain_params &p, const ChainPlan &c,
+                        size_t pi, int32_t i)
+{
+    const Proj q = c.proj[pi];
+    const int64_t s0 = c.pops[q.src].first;
+    const int64_t d0 = c.pops[q.dst].first;
+    const int32_t dn = c.pops[q.dst].second;
+    const int32_t f = q.f;
+    const bool is_drive = c.drive && pi == 0;
+    const uint64_t sid = (uint64_t)pi;
+    for (int32_t k = 0; k < f; k++) {
+        const int64_t e = c.proj_off[pi] + (int64_t)i * f + k;
+        const uint64_t cnt = (uint64_t)i * (uint64_t)f + (uint64_t)k;
+        const int64_t lo = ((int64_t)k * dn) / f, hi = ((int64_t)(k + 1) * dn) / f;
+        int64_t off = lo + (int64_t)(u01(h3(p.seed, (TAG_TGT << 32) | sid, cnt)) * (double)(hi - lo));
+        if (off >= hi) off = hi - 1;
+        h.pre[e] = (int32_t)(s0 + i);
+        h.post[e] = (int32_t)(d0 + off);
+        float w;
+        int8_t inh = 0;
+        if (is_drive) {
+            w = p.drive_w;
+        } else {
+            inh = u01(h3(p.seed, (TAG_INH << 32) | sid, cnt)) < (double)p.inh_frac;
+            const double u = u01(h3(p.seed, (TAG_W << 32) | sid, cnt));
+            w = inh ? (float)(-0.5 * (double)p.w0 * u) : (float)(0.7 * (double)p.w0 * u);
+        }
+        int32_t dk = 1 + (int32_t)(u01(h3(p.seed, (TAG_D << 32) | sid, cnt)) * p.dmax);
+        if (dk > p.dmax) dk = p.dmax;
+        h.w[e] = w;
+        h.d[e] = (float)dk * p.dt;
+        h.stype[e] = inh;
+    }
+}
+
+}  // namespace
+
+extern "C" int32_t bsim_net_gen_chains(bsim_network *netp, const bsim_chain_params *params,
+                                       int32_t count, int32_t *first_ids)
+{
+    if (!netp || !params || count < 0) {
+        g_bsim_err = "bsim_net_gen_chains: bad arguments";
         return BSIM_E_ARG;
     }
-    const bsim_chain_params p = *gp;
-    bsim::HostNet &h = netp->net;
-    const int64_t first = h.n();
-    int64_t nn = p.n_drive;
-    for (int32_t i = 0; i < p.npop; i++) {
-        if (p.sizes[i] < 1) {
-            g_bsim_err = "bsim_net_gen_chain: empty population";
+    for (int32_t c = 0; c < count; c++)
+        if (!chain_valid(params[c])) {
+            g_bsim_err = "bsim_net_gen_chain: bad parameters (chain " + std::to_string(c) + ")";
             return BSIM_E_ARG;
         }
-        nn += p.sizes[i];
-    }
-    if (first + nn > 0x7FFFFFFF) {
-        g_bsim_err = "too many neurons";
-        return BSIM_E_ARG;
-    }
-    const bool drive = p.n_drive > 0 && p.drive_rate_hz > 0 && p.drive_fan_in > 0;
+    bsim::HostNet &h = netp->net;
     try {
-        /* neurons: drive sources (keys 0.. of the chain's own seed), then the
-         * populations; every draw is keyed by chain-local indices */
-        std::vector<std::pair<int64_t, int32_t>> pops;   /* (global start, count) */
-        if (p.n_drive > 0) {
-            const double pr = std::min(1.0, (double)p.drive_rate_hz * (double)p.dt);
-            const uint32_t thr = (uint32_t)std::min(4294967295.0, std::round(pr * 4294967296.0));
-            h.kind.insert(h.kind.end(), (size_t)p.n_drive, (int8_t)BSIM_KIND_BERNOULLI);
-            for (int32_t i = 0; i < p.n_drive; i++) {
-                float row[bsim::kLifParams] = {0};
-                row[10] = 1e-3f;
-                row[11] = 1e-3f;
-                h.lif.insert(h.lif.end(), row, row + bsim::kLifParams);
-                h.bern_key.push_back((uint32_t)i);
+        std::vector<ChainPlan> plan((size_t)count);
+        int64_t n0 = h.n(), s0 = h.s();
+        int64_t n = n0, s = s0;
+        for (int32_t c = 0; c < count; c++) {
+            plan[c] = plan_chain(params[c], n, s);
+            n += plan[c].nn;
+            s += plan[c].nsyn;
+            if (n > 0x7FFFFFFF) {
+                g_bsim_err = "too many neurons";
+                return BSIM_E_ARG;
             }
-            h.bern_p.insert(h.bern_p.end(), (size_t)p.n_drive, drive ? thr : 0u);
-            h.bern_seed.insert(h.bern_seed.end(), (size_t)p.n_drive, p.seed);
-            h.arr_off.insert(h.arr_off.end(), (size_t)p.n_drive, h.arr_off.back());
-            h.seed = p.seed;
         }
-        const int64_t lif0 = first + p.n_drive;
-        const int64_t nlif = nn - p.n_drive;
-        const size_t base = h.kind.size();
-        h.kind.insert(h.kind.end(), (size_t)nlif, (int8_t)BSIM_KIND_LIF);
-        h.lif.resize((base + (size_t)nlif) * bsim::kLifParams);
-        h.bern_p.insert(h.bern_p.end(), (size_t)nlif, 0u);
-        h.bern_seed.insert(h.bern_seed.end(), (size_t)nlif, 0ull);
-        h.bern_key.insert(h.bern_key.end(), (size_t)nlif, 0u);
-        h.arr_off.insert(h.arr_off.end(), (size_t)nlif, h.arr_off.back());
-#pragma omp parallel for schedule(static)
-        for (int64_t g = 0; g < nlif; g++) {
-            float *row = &h.lif[(base + (size_t)g) * bsim::kLifParams];
-            float ioff = 0.0f;
-            if (p.ioff_hi > p.ioff_lo)
-                ioff = (float)((double)p.ioff_lo + ((double)p.ioff_hi - (double)p.ioff_lo) *
-                                                       u01(h3(p.seed, TAG_IOFF << 32, (uint64_t)g)));
-            const float r[bsim::kLifParams] = {0.0f, 0.0f, 0.0f, 20e-3f, 20e-3f, p.tau_refrac,
-                                               p.tau_syn_E, p.tau_syn_I, p.v_thresh, ioff,
-                                               p.tau_syn_E, p.tau_syn_I};
-            std::copy(r, r + bsim::kLifParams, row);
-        }
-        if (p.n_drive > 0) pops.push_back({first, p.n_drive});
-        int64_t at = lif0;
-        for (int32_t i = 0; i < p.npop; i++) {
-            pops.push_back({at, p.sizes[i]});
-            at += p.sizes[i];
-        }
-        const int32_t pl = p.n_drive > 0 ? 1 : 0;   /* index of population 0 in pops */
-
-        /* projections drive -> 0 and i -> i+1; per pre a fan-out of
-         * round(fan_in * n_dst / n_src) distinct targets by stratified
-         * sampling, pre-major (ascending pre for every target) */
-        std::vector<Proj> proj;
-        if (drive) {
-            const double k = std::round((double)p.drive_fan_in * p.sizes[0] / (double)p.n_drive);
-            proj.push_back({0, pl, (int32_t)std::max(1.0, std::min(k, (double)p.sizes[0]))});
-        }
-        for (int32_t i = 0; i + 1 < p.npop && p.fan_in > 0; i++) {
-            const double k = std::round((double)p.fan_in * p.sizes[i + 1] / (double)p.sizes[i]);
-            proj.push_back({pl + i, pl + i + 1, (int32_t)std::max(1.0, std::min(k, (double)p.sizes[i + 1]))});
-        }
-        int64_t total = 0;
-        for (const Proj &q : proj) total += (int64_t)pops[q.src].second * q.f;
-        const int64_t o0 = h.s();
-        h.pre.resize(o0 + total);
-        h.post.resize(o0 + total);
-        h.w.resize(o0 + total);
-        h.d.resize(o0 + total);
-        h.stype.resize(o0 + total);
-        if (!h.tau.empty()) h.tau.resize(o0 + total, 0.0f);
-        int64_t o = o0;
-        for (size_t pi = 0; pi < proj.size(); pi++) {
-            const Proj q = proj[pi];
-            const int64_t s0 = pops[q.src].first;
-            const int32_t sn = pops[q.src].second;
-            const int64_t d0 = pops[q.dst].first;
-            const int32_t dn = pops[q.dst].second;
-            const int32_t f = q.f;
-            const bool is_drive = drive && pi == 0;
-            const uint64_t sid = (uint64_t)pi;
-#pragma omp parallel for schedule(static)
-            for (int32_t i = 0; i < sn; i++) {
-                for (int32_t k = 0; k < f; k++) {
-                    const int64_t e = o + (int64_t)i * f + k;
-                    const uint64_t c = (uint64_t)i * (uint64_t)f + (uint64_t)k;
-                    const int64_t lo = ((int64_t)k * dn) / f, hi = ((int64_t)(k + 1) * dn) / f;
-                    int64_t off = lo + (int64_t)(u01(h3(p.seed, (TAG_TGT << 32) | sid, c)) * (double)(hi - lo));
-                    if (off >= hi) off = hi - 1;
-                    h.pre[e] = (int32_t)(s0 + i);
-                    h.post[e] = (int32_t)(d0 + off);
-                    float w;
-                    int8_t inh = 0;
-                    if (is_drive) {
-                        w = p.drive_w;
-                    } else {
-                        inh = u01(h3(p.seed, (TAG_INH << 32) | sid, c)) < (double)p.inh_frac;
-                        const double u = u01(h3(p.seed, (TAG_W << 32) | sid, c));
-                        w = inh ? (float)(-0.5 * (double)p.w0 * u) : (float)(0.7 * (double)p.w0 * u);
-                    }
-                    int32_t dk = 1 + (int32_t)(u01(h3(p.seed, (TAG_D << 32) | sid, c)) * p.dmax);
-                    if (dk > p.dmax) dk = p.dmax;
-                    h.w[e] = w;
-                    h.d[e] = (float)dk * p.dt;
-                    h.stype[e] = inh;
-                }
+        if (count == 0) return (int32_t)n0;
+        /* every array sized once; all chains filled in parallel */
+        const int64_t arr_end = h.arr_off.back();
+        h.kind.resize(n);
+        h.lif.resize((size_t)n * bsim::kLifParams);
+        h.bern_p.resize(n);
+        h.bern_seed.resize(n);
+        h.bern_key.resize(n);
+        h.arr_off.resize((size_t)n + 1);
+        h.pre.resize(s);
+        h.post.resize(s);
+        h.w.resize(s);
+        h.d.resize(s);
+        h.stype.resize(s);
+        if (!h.tau.empty()) h.tau.resize(s, 0.0f);
+#pragma omp parallel for schedule(dynamic, 1)
+        for (int32_t c = 0; c < count; c++) write_chain_neurons(h, params[c], plan[c], arr_end);
+        /* work items: (chain, projection), each a run of pres */
+        std::vector<std::pair<int32_t, int32_t>> item;
+        std::vector<int64_t> item_pre0(1, 0);
+        for (int32_t c = 0; c < count; c++)
+            for (size_t pi = 0; pi < plan[c].proj.size(); pi++) {
+                item.push_back({c, (int32_t)pi});
+                item_pre0.push_back(item_pre0.back() + plan[c].pops[plan[c].proj[pi].src].second);
             }
-            o += (int64_t)sn * f;
+        const int64_t npre = item_pre0.back();
+#pragma omp parallel for schedule(dynamic, 256)
+        for (int64_t x = 0; x < npre; x++) {
+            const size_t it = (size_t)(std::upper_bound(item_pre0.begin(), item_pre0.end(), x) -
+                                       item_pre0.begin()) - 1;
+            const int32_t c = item[it].first, pi = item[it].second;
+            write_pre_synapses(h, params[c], plan[c], (size_t)pi, (int32_t)(x - item_pre0[it]));
         }
+        for (int32_t c = 0; c < count; c++)
+            if (params[c].n_drive > 0) h.seed = params[c].seed;
+        if (first_ids)
+            for (int32_t c = 0; c < count; c++) first_ids[c] = (int32_t)plan[c].first;
+        return (int32_t)n0;
     } catch (...) {
         g_bsim_err = "out of host memory";
         return BSIM_E_NOMEM;
     }
-    return (int32_t)first;
+}
+
+extern "C" int32_t bsim_net_gen_chain(bsim_network *netp, const bsim_chain_params *gp)
+{
+    if (!netp || !gp || !chain_valid(*gp)) {
+        g_bsim_err = "bsim_net_gen_chain: bad parameters";
+        return BSIM_E_ARG;
+    }
+    int32_t first = 0;
+    const int32_t r = bsim_net_gen_chains(netp, gp, 1, &first);
+    return r < 0 ? r : first;
 }
 
 extern "C" int32_t bsim_net_export_neurons(const bsim_network *netp, int8_t *kind, float *lif,

@@ -11,6 +11,7 @@
 
 #include <algorithm>
 #include <map>
+#include <memory>
 #include <cstring>
 #include <numeric>
 
@@ -192,7 +193,18 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
      * stable counting sort (per-thread counts, prefix over (tile, thread)) */
     const int32_t NT = L.ntiles;
     L.tile_syn.assign((size_t)NT + 1, 0);
-    std::vector<int64_t> order(S);
+    /* creation index of each synapse in tile-bucket order (not value-
+     * initialised: every entry is written below) */
+    std::unique_ptr<int64_t[]> order_buf(new int64_t[(size_t)std::max<int64_t>(S, 1)]);
+    int64_t *order = order_buf.get();
+    /* post / tile without a division: a double reciprocal, corrected */
+    const double inv_tile = 1.0 / (double)tile;
+    auto tile_of = [tile, inv_tile](int32_t post) {
+        int64_t t = (int64_t)((double)post * inv_tile);
+        if (t * tile > post) t--;
+        else if ((t + 1) * tile <= post) t++;
+        return (int32_t)t;
+    };
     {
         const int nmax = omp_get_max_threads();
         std::vector<int64_t> cnt((size_t)nmax * NT, 0);
@@ -204,7 +216,7 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
             nth = T;
             const int64_t k0 = S * id / T, k1 = S * (id + 1) / T;
             int64_t *c = cnt.data() + (size_t)id * NT;
-            for (int64_t k = k0; k < k1; k++) c[net.post[k] / tile]++;
+            for (int64_t k = k0; k < k1; k++) c[tile_of(net.post[k])]++;
 #pragma omp barrier
 #pragma omp single
             {
@@ -219,7 +231,7 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
                 }
                 L.tile_syn[NT] = run;
             }
-            for (int64_t k = k0; k < k1; k++) order[c[net.post[k] / tile]++] = k;
+            for (int64_t k = k0; k < k1; k++) order[c[tile_of(net.post[k])]++] = k;
         }
     }
     for (int32_t t = 0; t < NT; t++) {

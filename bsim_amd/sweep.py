@@ -133,7 +133,7 @@ class Batch:
 def build(points):
     """All points appended to one host network."""
     net = api.Network()
-    first = np.array([net.add_chain(**p.kwargs()) for p in points], np.int64)
+    first = net.add_chains([p.kwargs() for p in points])
     return Batch(net=net, points=list(points), first=first)
 
 

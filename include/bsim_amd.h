@@ -227,6 +227,13 @@ typedef struct bsim_chain_params {
  * depend only on p, so the network behaves in a batch exactly as alone. */
 int32_t bsim_net_gen_chain(bsim_network *net, const bsim_chain_params *p);
 
+/* Appends count sweep networks, the same network as count calls of
+ * bsim_net_gen_chain in that order, with the host arrays sized once and all
+ * chains generated in parallel; first_ids[c] (if not NULL) receives chain c's
+ * first neuron id.  Returns the id of the first chain's first neuron. */
+int32_t bsim_net_gen_chains(bsim_network *net, const bsim_chain_params *p, int32_t count,
+                            int32_t *first_ids);
+
 /* Copies out the network as stored (creation order); NULL arrays skipped.
  * lif receives n * 12 floats. */
 int32_t bsim_net_export_neurons(const bsim_network *net, int8_t *kind, float *lif,

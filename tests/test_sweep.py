@@ -33,6 +33,23 @@ def test_chain_is_the_same_alone_and_in_a_batch():
         assert np.array_equal(full.stype[sel], a.stype)
 
 
+def test_batched_generator_equals_chain_by_chain():
+    """bsim_net_gen_chains builds the same network as bsim_net_gen_chain called
+    once per chain, appended after an existing chain."""
+    pts = sweep.sample(6, seed=4, **SMALL)
+    a, b = api.Network(), api.Network()
+    a.add_chain(**pts[0].kwargs())
+    b.add_chain(**pts[0].kwargs())
+    fa = [a.add_chain(**p.kwargs()) for p in pts[1:]]
+    fb = b.add_chains([p.kwargs() for p in pts[1:]])
+    assert list(fa) == list(fb)
+    sa, sb = a.export_spec(), b.export_spec()
+    assert sa.seed == sb.seed
+    for f in ("kind", "lif", "bern_p", "pre", "post", "w", "d", "stype"):
+        x, y = np.ascontiguousarray(getattr(sa, f)), np.ascontiguousarray(getattr(sb, f))
+        assert x.shape == y.shape and np.array_equal(x.view(np.uint8), y.view(np.uint8)), f
+
+
 def test_chain_respects_fan_in_and_pre_order():
     p = sweep.sample(1, seed=5, **SMALL)[0]
     net = api.Network()
