@@ -94,6 +94,28 @@ def make_workload(name):
     raise SystemExit(f"unknown workload {name}")
 
 
+def cpu_reference(workload, steps):
+    """The reference's own CPU simulator (STSim, float build, compiled from
+    its sources into oracle/_ref) timed on a bounded sample: the workload's
+    generator with 2 populations (its cost is O(synapses) per cycle, so the
+    full network is out of reach), events counted by the oracle over the same
+    cycles.  None when oracle/_ref is not built."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import oracle  # noqa: E402
+    import ref_io  # noqa: E402
+    if ref_io.ref_binary() is None or workload not in ("c1", "c2"):
+        return None
+    params = netspec.C1 if workload == "c1" else netspec.C2
+    spec = api.Network.standard(**{**params, "npop": 2}).export_spec()
+    secs = ref_io.time_ref(spec, netspec.DT, steps)
+    ev = oracle.run(spec, netspec.DT, steps).events
+    return {"value": ev / secs, "unit": "synaptic events/s", "cores": 1, "kind": "reference",
+            "sample": f"reference STSim::run (oracle/_ref, float) on the same generator with 2 "
+                      f"populations ({len(spec.pre)} synapses), cycles 0..{steps}: {ev} events "
+                      f"in {secs:.2f}s ({secs / steps * 1e3:.1f} ms per cycle: every synapse "
+                      f"is visited every cycle)"}
+
+
 def cpu_baseline(spec, warmup, steps, threads):
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # noqa: E402  (the oracle is only the checker / baseline)
@@ -388,6 +410,8 @@ def main():
     ap.add_argument("--tile", type=int, default=0)
     ap.add_argument("--cpu-steps", type=int, default=100)
     ap.add_argument("--cpu-threads", type=int, default=0)
+    ap.add_argument("--ref-steps", type=int, default=100,
+                    help="cycles of the reference CPU simulator's timed sample")
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--roofline-steps", type=int, default=200)
     ap.add_argument("--copy-probe", action="store_true")
@@ -474,6 +498,7 @@ def main():
     sim.close()
 
     cpu = None
+    cpu_ref = None
     if not a.no_cpu:
         threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
         cw = min(a.warmup, 1000)
@@ -483,6 +508,10 @@ def main():
                          f"network ({ev} events in {secs:.2f}s)", "host": host_info()}
         print(f"[bench] cpu baseline {v:.3g} events/s on {threads} threads", file=sys.stderr,
               flush=True)
+        cpu_ref = cpu_reference(a.workload, a.ref_steps)
+        if cpu_ref:
+            print(f"[bench] reference CPU path {cpu_ref['value']:.3g} events/s ({cpu_ref['sample']})",
+                  file=sys.stderr, flush=True)
 
     out = {
         "metric": "synaptic events/s",
@@ -507,6 +536,7 @@ def main():
                      "events_per_launch": ev_per_launch,
                      "row_list_kernel_us": rows_us},
         "cpu_baseline": cpu,
+        "cpu_reference": cpu_ref,
     }
     print(json.dumps(out), flush=True)
 

@@ -288,10 +288,8 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
         fail(BSIM_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
         return nullptr;
     }
-    int cus = 256;
-    hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, c.device);
     const int32_t N = net->net.n();
-    const int32_t tile = c.tile > 0 ? c.tile : bsim::default_tile(N, cus);
+    const int32_t tile = c.tile > 0 ? c.tile : bsim::default_tile(bsim::max_delay_steps(net->net, dt));
     for (int32_t r = 0; r < c.n_rec; r++) {
         if (c.rec_ids[r] < 0 || c.rec_ids[r] >= N) {
             fail(BSIM_E_ARG, "recorded neuron id out of range");

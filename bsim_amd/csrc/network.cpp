@@ -70,15 +70,29 @@ bool synapse_reset(float w, float delay, float tau, float dt, float *w_out, int3
     return true;
 }
 
-int32_t default_tile(int32_t n, int32_t cus)
+int32_t max_delay_steps(const HostNet &net, float dt)
 {
-    /* The smallest tile: the most workgroups to balance uneven activity over
-     * the compute units, one accumulator slot per lane.  Larger tiles fetch
-     * fewer row-table lines per active row but leave heavy tiles on few
-     * wavefronts. */
-    (void)n;
-    (void)cus;
-    return BSIM_TILE_ALIGN;
+    const int64_t S = net.s();
+    int32_t dmax = 1;
+#pragma omp parallel for schedule(static) reduction(max : dmax)
+    for (int64_t k = 0; k < S; k++) {
+        const int32_t steps = (int32_t)roundf(net.d[k] / dt);
+        if (steps > dmax) dmax = steps;
+    }
+    return dmax;
+}
+
+int32_t default_tile(int32_t dmax)
+{
+    /* Each active (pre, delay) row costs one row-table vector per tile it
+     * reaches; a row's synapses spread over more delays leave fewer of them
+     * per tile, so deep delay rings take larger tiles (fewer, fuller rows per
+     * tile) while shallow ones keep the smallest tile (most workgroups to
+     * balance uneven activity).  Measured: D = 6 best at 192, D = 6..20
+     * sweeps at 192..384, D = 50 at 768. */
+    if (dmax <= 8) return BSIM_TILE_ALIGN;
+    if (dmax <= 24) return 2 * BSIM_TILE_ALIGN;
+    return 4 * BSIM_TILE_ALIGN;
 }
 
 /* Weight and delay steps of synapse k (StaticSynapse::reset); 0, or a

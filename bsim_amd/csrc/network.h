@@ -83,7 +83,9 @@ void lif_constants(const float *p, float dt, float out[9], int32_t *refrac_time)
 bool synapse_reset(float w, float delay, float tau, float dt, float *w_out, int32_t *ds);
 
 /* Chooses the tile size for n neurons on a device with `cus` compute units. */
-int32_t default_tile(int32_t n, int32_t cus);
+int32_t default_tile(int32_t dmax);
+/* Largest round(delay / dt) over the synapses (at least 1). */
+int32_t max_delay_steps(const HostNet &net, float dt);
 
 /* Returns 0 or a negative code with a message in err. */
 int build_layout(const HostNet &net, float dt, int32_t tile, Layout &out, std::string &err);

@@ -108,7 +108,8 @@ int64_t bsim_net_num_synapses(const bsim_network *net);
 
 typedef struct bsim_sim_config {
     int32_t device;        /* HIP device ordinal */
-    int32_t tile;          /* neurons per tile (multiple of 192, at most 1536); 0 = automatic */
+    int32_t tile;          /* neurons per tile (multiple of 192, at most 1536); 0 = automatic:
+                              192 for delays up to 8 steps, 384 up to 24, else 768 */
     int32_t rec_steps;     /* cycles of raster/trace kept on the device (0 = none) */
     int32_t n_rec;         /* neurons whose V is traced */
     const int32_t *rec_ids;
