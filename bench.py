@@ -408,7 +408,7 @@ def main():
     ap.add_argument("--warmup", type=int, default=1000)
     ap.add_argument("--workload", default="c1")
     ap.add_argument("--tile", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=100)
+    ap.add_argument("--cpu-steps", type=int, default=1000)
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--ref-steps", type=int, default=100,
                     help="cycles of the reference CPU simulator's timed sample")

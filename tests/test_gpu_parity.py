@@ -114,6 +114,26 @@ def test_standard_medium_vs_csr_oracle():
     assert_same(r, o, "standard-medium")
 
 
+@pytest.mark.parametrize("tile", [0, 192])
+def test_stress_shape_vs_csr_oracle(tile):
+    """The high-activity stress shape at 1/100 scale (10 x 1000, fan-out 200
+    with w0 x 10 to keep the recurrent input of fan-out 2,000, delays 1..50,
+    no refractory period): activity saturates (about 30 % of the neurons fire
+    every cycle at the end), so much of the 50-deep history is live.  GPU at
+    the automatic tile (768 for delays over 24 steps) and at 192 vs the
+    oracle's CSR form."""
+    spec = netspec.c2(npop=10, n=1000, fanout=200, n_drive=1000, seed=21, w0=0.45 * 1.3 * 10)
+    steps = 600
+    rec = [1000, 5000, 10999]
+    say(f"[gpu] stress shape n={spec.n} s={spec.s} tile={tile}")
+    r = run_gpu(spec, steps, rec, tile)
+    o = oracle.run(spec, DT, steps, rec_ids=rec, mode="csr", tile=512, threads=8)
+    assert r["info"]["dmax"] == 50
+    assert r["info"]["tile"] == (768 if tile == 0 else tile)
+    assert o.events > 50_000_000
+    assert_same(r, o, "stress-shape")
+
+
 def test_run_twice_identical():
     spec = netspec.standard(npop=4, n=500, fanout=80, w0=3.0, drive_rate_hz=400.0, n_drive=200,
                             drive_w=12.0, seed=3, i_offset=(11.0, 14.0))
