@@ -50,6 +50,8 @@ _SIGS = {
     "bsim_sim_read_raster": (C.c_int64, [P, P, P, C.c_int64, P]),
     "bsim_sim_read_trace": (C.c_int32, [P, P]),
     "bsim_sim_read_state": (C.c_int32, [P, P, P, P, P, P]),
+    "bsim_sim_enable_input_log": (C.c_int32, [P]),
+    "bsim_sim_read_input_log": (C.c_int32, [P, P, P]),
     "bsim_sim_events": (C.c_int64, [P]),
     "bsim_sim_time_steps": (C.c_int32, [P, C.c_int32, P, P]),
     "bsim_sim_get_info": (C.c_int32, [P, P]),
@@ -416,6 +418,21 @@ class Simulation:
         if r < 0:
             _err(r, "bsim_sim_time_steps")
         return a.value, b.value
+
+    def enable_input_log(self):
+        """FireInfo "Y" / "X" logs from the next cycle on (bsim_sim_enable_input_log)."""
+        r = lib().bsim_sim_enable_input_log(self._h)
+        if r < 0:
+            _err(r, "bsim_sim_enable_input_log")
+
+    def input_log(self):
+        """(Y: summed LIF input per neuron, X: delay slots visited per neuron)."""
+        y = np.zeros(self.n, np.float32)
+        x = np.zeros(self.n, np.int32)
+        r = lib().bsim_sim_read_input_log(self._h, _p(y), _p(x))
+        if r < 0:
+            _err(r, "bsim_sim_read_input_log")
+        return y, x
 
     def events(self):
         r = lib().bsim_sim_events(self._h)

@@ -40,6 +40,8 @@ struct bsim_sim {
     hipGraphExec_t gexec = nullptr;
     int32_t rec_steps = 0, n_rec = 0;
     int64_t n_src = 0, n_off = 0, s = 0;
+    float *log_in = nullptr;       /* input logs (bsim_sim_enable_input_log) */
+    int32_t *log_slot = nullptr;
 };
 
 extern "C" {
@@ -620,6 +622,49 @@ int32_t bsim_sim_read_state(bsim_sim *sim, float *v, float *ie, float *ii, int32
         (e = fetch(sim, refrac, sim->d.refrac, n)) != hipSuccess ||
         (e = fetch(sim, fire_count, sim->d.fire_count, n)) != hipSuccess)
         return fail(BSIM_E_HIP, std::string("state copy: ") + hipGetErrorString(e));
+    return 0;
+}
+
+int32_t bsim_sim_enable_input_log(bsim_sim *sim)
+{
+    if (!sim) return fail(BSIM_E_ARG, "bad argument");
+    if (sim->log_in) return 0;
+    hipSetDevice(sim->device);
+    const size_t n = (size_t)(sim->d.n > 0 ? sim->d.n : 1);
+    hipError_t e = hipSuccess;
+    float *in = dev_alloc<float>(sim, n, e);
+    if (e != hipSuccess) return fail(BSIM_E_NOMEM, std::string("input log: ") + hipGetErrorString(e));
+    int32_t *slot = dev_alloc<int32_t>(sim, n, e);
+    if (e != hipSuccess) return fail(BSIM_E_NOMEM, std::string("input log: ") + hipGetErrorString(e));
+    if ((e = hipMemsetAsync(in, 0, n * sizeof(float), sim->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(slot, 0, n * sizeof(int32_t), sim->stream)) != hipSuccess)
+        return fail(BSIM_E_HIP, std::string("input log: ") + hipGetErrorString(e));
+    sim->log_in = in;
+    sim->log_slot = slot;
+    sim->d.in_sum = in;
+    /* without synapses the reference's delay ring has no slot (MAX_DELAY 0) */
+    sim->d.slot_count = sim->s > 0 ? slot : nullptr;
+    /* the captured launches hold the kernel arguments: capture again */
+    if (sim->gexec) {
+        hipStreamSynchronize(sim->stream);
+        hipGraphExecDestroy(sim->gexec);
+        hipGraphDestroy(sim->graph);
+        sim->gexec = nullptr;
+        sim->graph = nullptr;
+    }
+    return 0;
+}
+
+int32_t bsim_sim_read_input_log(bsim_sim *sim, float *in_sum, int32_t *slot_count)
+{
+    if (!sim) return fail(BSIM_E_ARG, "bad argument");
+    if (!sim->log_in) return fail(BSIM_E_STATE, "input log not enabled (bsim_sim_enable_input_log)");
+    hipSetDevice(sim->device);
+    const size_t n = (size_t)sim->d.n;
+    hipError_t e;
+    if ((e = fetch(sim, in_sum, sim->log_in, n)) != hipSuccess ||
+        (e = fetch(sim, slot_count, sim->log_slot, n)) != hipSuccess)
+        return fail(BSIM_E_HIP, std::string("input log copy: ") + hipGetErrorString(e));
     return 0;
 }
 

@@ -126,6 +126,14 @@ typedef struct bsim_dev {
     int32_t pad1;
     const int32_t *rec_index;   /* [n]: trace column of a neuron or -1 (NULL: none) */
     float *rec_v;               /* [rec_steps][n_rec] */
+    /* input logs (NULL: off).  in_sum[n]: a LIF neuron's E + I input summed
+     * over the cycles it neither was refractory nor fired (the reference GPU
+     * simulator's gXInput, src/neuron/lif/lif.cu:116); slot_count[n]: the
+     * delay slots of the neuron's spikes visited by the deliveries, i.e. the
+     * number of bits 1..D of its history summed over cycles (gLayerInput,
+     * src/synapse/static/static.cu:42) */
+    float *in_sum;
+    int32_t *slot_count;
 } bsim_dev;
 
 #endif

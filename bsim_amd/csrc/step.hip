@@ -518,6 +518,7 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
                             d.refrac[nid] = d.refrac_time[nid] - 1;
                             v = d.v_reset[nid];
                         } else {
+                            if (d.in_sum) d.in_sum[nid] = d.in_sum[nid] + (inE + inI);
                             ie += inE;
                             ii += inI;
                         }
@@ -536,6 +537,7 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
                 }
                 h = (hist_in[nid] << 1) | (fired ? 1ull : 0ull);
                 hist_out[nid] = h;
+                if (d.slot_count) d.slot_count[nid] += __popcll((h >> 1) & dmask);
                 if (fired) d.fire_count[nid] += 1;
                 if (rec && d.rec_index) {
                     const int32_t ri = d.rec_index[nid];

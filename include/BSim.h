@@ -42,8 +42,10 @@
  *   - Output files: GSim.info always; GSim.log (fired ids per cycle, ascending)
  *     and g_v.data (V of the monitored neurons per cycle, "%.10lf \t") when
  *     enabled with setLog / monitor (the reference writes them under
- *     -DLOG_DATA, src/sim/SingleGPUSimulator.cu:176-232).  FireInfo "count"
- *     is filled; "X" and "Y" are returned empty (size 0).
+ *     -DLOG_DATA, src/sim/SingleGPUSimulator.cu:176-232).  FireInfo "count",
+ *     "Y" (summed LIF input) and "X" (delay slots visited) are filled as the
+ *     reference GPU simulator fills them (src/sim/SingleGPUSimulator.cu:263-296);
+ *     run(T) writes GFire.count (src/base/SimulatorBase.cpp:29-37).
  */
 #ifndef BSIM_AMD_BSIM_H
 #define BSIM_AMD_BSIM_H
@@ -382,10 +384,27 @@ public:
     void monitor(const std::vector<int> &gids) { monitor_ = gids; }
     void setDevice(int device) { device_ = device; }
 
+    /* SimulatorBase::run (src/base/SimulatorBase.cpp:29-37): runs with a
+     * "count" log and writes it to GFire.count ("%d " per neuron, then a
+     * newline, as logFireInfo in src/base/SimInfo.cpp:4-33 does). */
     int run(real time)
     {
         FireInfo log;
-        return run(time, log);
+        ArrayInfo zero = {0, NULL};
+        log["count"] = zero;
+        const int ret = run(time, log);
+        FILE *f = fopen("GFire.count", "w+");
+        if (!f) {
+            printf("ERROR: Open file GFire.count failed\n");
+            free(log["count"].data);
+            return -1;
+        }
+        const int *c = reinterpret_cast<const int *>(log["count"].data);
+        for (int j = 0; j < log["count"].size; j++) fprintf(f, "%d ", c[j]);
+        fprintf(f, "\n");
+        fclose(f);
+        free(log["count"].data);
+        return ret;
     }
 
     int run(real time, FireInfo &log)
@@ -406,6 +425,8 @@ public:
         } guard = {sim};
         bsim_sim_info info;
         bsim_cpp::check(bsim_sim_get_info(sim, &info), "SGSim");
+        const bool want_y = log.find("Y") != log.end(), want_x = log.find("X") != log.end();
+        if (want_y || want_x) bsim_cpp::check(bsim_sim_enable_input_log(sim), "SGSim input log");
 
         FILE *inf = fopen("GSim.info", "w+");
         if (!inf) {
@@ -436,13 +457,23 @@ public:
             log["count"].size = info.n;
             log["count"].data = count;
         }
-        if (log.find("Y") != log.end()) {
-            log["Y"].size = 0;
-            log["Y"].data = NULL;
-        }
-        if (log.find("X") != log.end()) {
-            log["X"].size = 0;
-            log["X"].data = NULL;
+        if (want_y || want_x) {
+            const size_t n = (size_t)(info.n > 0 ? info.n : 1);
+            real *Y = (real *)malloc(sizeof(real) * n);
+            int *X = (int *)malloc(sizeof(int) * n);
+            bsim_cpp::check(bsim_sim_read_input_log(sim, Y, X), "SGSim input log");
+            if (want_y) {
+                log["Y"].size = info.n;
+                log["Y"].data = Y;
+            } else {
+                free(Y);
+            }
+            if (want_x) {
+                log["X"].size = info.n;
+                log["X"].data = X;
+            } else {
+                free(X);
+            }
         }
         return 0;
     }

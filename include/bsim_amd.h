@@ -155,6 +155,17 @@ int32_t bsim_sim_read_state(bsim_sim *sim, float *v, float *ie, float *ii, int32
  * production path (graph-replayed, no events). */
 int32_t bsim_sim_time_steps(bsim_sim *sim, int32_t n, double *step_us, double *rows_us);
 
+/* Turns on the per-neuron input logs of the reference GPU simulator's
+ * FireInfo (src/sim/SingleGPUSimulator.cu:284-296): "Y" -- a LIF neuron's
+ * E + I input summed over the cycles it neither was refractory nor fired
+ * (gXInput, src/neuron/lif/lif.cu:116) -- and "X" -- the delay slots of the
+ * neuron's spikes visited by the deliveries (gLayerInput,
+ * src/synapse/static/static.cu:42, :82, :106).  Counting starts with the next
+ * cycle; call before the first bsim_sim_step to cover the whole run. */
+int32_t bsim_sim_enable_input_log(bsim_sim *sim);
+/* Copies the logs (n values each; NULL arrays skipped). */
+int32_t bsim_sim_read_input_log(bsim_sim *sim, float *in_sum, int32_t *slot_count);
+
 /* Synaptic deliveries performed so far (sum over tiles). */
 int64_t bsim_sim_events(bsim_sim *sim);
 

@@ -84,6 +84,13 @@ typedef struct bso_out_##SUF {                                                \
     int32_t time_from;       /* cycle from which wall time is measured      */ \
     double wall_s;           /* seconds spent in cycles >= time_from        */ \
     int64_t events_from;     /* deliveries in cycles >= time_from           */ \
+    /* GPU simulator logs (FireInfo "Y", "X"; may be NULL) */                 \
+    REAL *input_sum;         /* [n] LIF input E + I summed over the cycles   */ \
+                             /* it neither was refractory nor fired         */ \
+                             /* (src/neuron/lif/lif.cu:116)                  */ \
+    int32_t *slot_count;     /* [n] delay slots of the neuron's spikes that  */ \
+                             /* the deliveries visited                       */ \
+                             /* (src/synapse/static/static.cu:42, :82, :106) */ \
 } bso_out_##SUF;                                                              \
                                                                               \
 int bso_lif_constants_##SUF(const bso_net_##SUF *net, REAL dt, REAL *out9,   \

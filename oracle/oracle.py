@@ -85,7 +85,7 @@ def _structs(suf):
                     ("rec_v", P), ("fire_count", P), ("final_v", P), ("final_ie", P),
                     ("final_ii", P), ("final_refrac", P), ("events", C.c_int64),
                     ("time_from", C.c_int32), ("wall_s", C.c_double),
-                    ("events_from", C.c_int64)]
+                    ("events_from", C.c_int64), ("input_sum", P), ("slot_count", P)]
 
     del real
     _STRUCTS[suf] = (Net, Out)
@@ -137,6 +137,10 @@ class OracleResult:
     final_ii: np.ndarray
     final_refrac: np.ndarray
     events: int
+    input_sum: np.ndarray = None   # [n] FireInfo "Y": LIF input E + I summed over the cycles
+                                   # it neither was refractory nor fired (lif.cu:116)
+    slot_count: np.ndarray = None  # [n] FireInfo "X": delay slots of its spikes visited by
+                                   # the deliveries (static.cu:42, :82, :106)
     wall_s: float = 0.0        # seconds spent in cycles >= time_from
     events_from: int = 0       # deliveries in cycles >= time_from
 
@@ -163,13 +167,15 @@ def run(net, dt, steps, rec_ids=(), dtype=np.float32, mode="exact", tile=4096, t
                rec_v=np.zeros((steps, len(rec)), a.dtype),
                fire_count=np.zeros(n, np.int32),
                final_v=np.zeros(n, a.dtype), final_ie=np.zeros(n, a.dtype),
-               final_ii=np.zeros(n, a.dtype), final_refrac=np.zeros(n, np.int32))
+               final_ii=np.zeros(n, a.dtype), final_refrac=np.zeros(n, np.int32),
+               input_sum=np.zeros(n, a.dtype), slot_count=np.zeros(n, np.int32))
     while True:
         out = Out(_ptr(res["raster_count"]), _ptr(res["raster_ids"]), raster_cap, 0,
                   len(rec), _ptr(rec) if len(rec) else None,
                   _ptr(res["rec_v"]) if len(rec) else None, _ptr(res["fire_count"]),
                   _ptr(res["final_v"]), _ptr(res["final_ie"]), _ptr(res["final_ii"]),
-                  _ptr(res["final_refrac"]), 0, int(time_from), 0.0, 0)
+                  _ptr(res["final_refrac"]), 0, int(time_from), 0.0, 0,
+                  _ptr(res["input_sum"]), _ptr(res["slot_count"]))
         dt_c = float(np.asarray(dt, a.dtype))
         if mode == "exact":
             rc = getattr(L, f"bso_run_{suf}")(C.byref(a.c), dt_c, steps, C.byref(out))

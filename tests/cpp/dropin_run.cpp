@@ -10,7 +10,10 @@
  *   int32 nproj; per projection: int32 src, dst, float w[n_src * n_dst],
  *   float d[n_src * n_dst]; float dt, T; int32 nmon, int32 gid[nmon].
  *
- * usage: dropin_run <input> <output dir>
+ * usage: dropin_run <input> <output dir> [plain]
+ *   default: run(T, log) with the FireInfo keys "count", "Y" and "X", written
+ *            to GFire.count, input_Y.txt (hex floats) and input_X.txt;
+ *   plain:   run(T), which writes GFire.count itself.
  */
 #include <stdio.h>
 #include <stdlib.h>
@@ -31,8 +34,8 @@ static void rd(FILE *f, T *p, size_t n)
 
 int main(int argc, char **argv)
 {
-    if (argc != 3) {
-        fprintf(stderr, "usage: %s <input> <output dir>\n", argv[0]);
+    if (argc != 3 && argc != 4) {
+        fprintf(stderr, "usage: %s <input> <output dir> [plain]\n", argv[0]);
         return 2;
     }
     FILE *f = fopen(argv[1], "rb");
@@ -86,8 +89,20 @@ int main(int argc, char **argv)
     SGSim sg(&c, dt);
     sg.setLog(true);
     sg.monitor(mon);
+    if (argc > 3) {   /* plain */
+        try {
+            if (sg.run(T) != 0) return 1;
+        } catch (const bsim_error &e) {
+            fprintf(stderr, "%s\n", e.what());
+            return 1;
+        }
+        printf("dropin_run ok\n");
+        return 0;
+    }
     FireInfo log;
     log["count"];
+    log["Y"];
+    log["X"];
     try {
         if (sg.run(T, log) != 0) return 1;
     } catch (const bsim_error &e) {
@@ -100,6 +115,16 @@ int main(int argc, char **argv)
     fprintf(o, "\n");
     fclose(o);
     free(cnt);
+    o = fopen("input_Y.txt", "w+");
+    real *y = (real *)log["Y"].data;
+    for (int i = 0; i < log["Y"].size; i++) fprintf(o, "%a\n", (double)y[i]);
+    fclose(o);
+    free(y);
+    o = fopen("input_X.txt", "w+");
+    int *x = (int *)log["X"].data;
+    for (int i = 0; i < log["X"].size; i++) fprintf(o, "%d\n", x[i]);
+    fclose(o);
+    free(x);
     printf("dropin_run ok\n");
     return 0;
 }

@@ -14,6 +14,7 @@ import numpy as np
 import pytest
 
 import golden_io
+import oracle
 from bsim_amd import netspec
 
 ROOT = os.path.dirname(os.path.dirname(os.path.abspath(__file__)))
@@ -77,3 +78,19 @@ def test_dropin_program_matches_reference_c0(tmp_path):
     np.testing.assert_allclose(gv, gold["rec_v"].astype(np.float64), rtol=0, atol=5.1e-11)
     info = (tmp_path / "GSim.info").read_text()
     assert "NeuronNum: 2000, SynapseNum: 1000000" in info
+    # FireInfo "Y" and "X" against the oracle (bit for bit)
+    o = oracle.run(spec, netspec.DT, steps)
+    y = np.array([float.fromhex(v) for v in (tmp_path / "input_Y.txt").read_text().split()],
+                 np.float32)
+    x = np.array((tmp_path / "input_X.txt").read_text().split(), np.int32)
+    assert np.array_equal(y.view(np.uint32), o.input_sum.view(np.uint32))
+    np.testing.assert_array_equal(x, o.slot_count)
+    assert x.sum() > 0 and np.count_nonzero(y) > 0
+    # run(T) writes GFire.count by itself
+    plain = tmp_path / "plain"
+    plain.mkdir()
+    r = subprocess.run([BIN, str(inp), str(plain), "plain"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    fc2 = np.array((plain / "GFire.count").read_text().split(), np.int32)
+    np.testing.assert_array_equal(fc2, gold["fire_count"])

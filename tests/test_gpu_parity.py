@@ -134,6 +134,41 @@ def test_stress_shape_vs_csr_oracle(tile):
     assert_same(r, o, "stress-shape")
 
 
+@pytest.mark.parametrize("seed", range(3))
+def test_input_logs_vs_oracle(seed):
+    """FireInfo "Y" (summed LIF input) and "X" (delay slots visited) from the
+    GPU vs the oracle, bit for bit: random networks with array and Bernoulli
+    sources and mixed delays, and a standard network."""
+    if seed < 2:
+        spec = netspec.random_net(seed + 40, n_lif=300, n_arr=20, n_bern=20, p_conn=0.1,
+                                  dmax=3 + 8 * seed, spike_prob=0.05, steps=300)
+    else:
+        spec = netspec.standard(npop=4, n=500, fanout=80, w0=3.0, drive_rate_hz=400.0,
+                                n_drive=100, seed=7, i_offset=(12.0, 14.9))
+    steps = 300
+    net = api.Network.from_spec(spec)
+    sim = api.Simulation(net, DT, tile=192)
+    sim.step(10)                       # graph-free cycles first, then the log turned on
+    sim.enable_input_log()
+    sim.step(steps - 10)
+    sim.finish()
+    sim.sync()
+    y_late, x_late = sim.input_log()
+    sim.close()
+    sim = api.Simulation(net, DT, tile=192)
+    sim.enable_input_log()
+    sim.step(steps)
+    sim.finish()
+    sim.sync()
+    y, x = sim.input_log()
+    sim.close()
+    o = oracle.run(spec, DT, steps)
+    assert np.array_equal(y.view(np.uint32), o.input_sum.view(np.uint32))
+    np.testing.assert_array_equal(x, o.slot_count)
+    assert x.sum() > 0 and np.count_nonzero(y) > 0
+    assert x_late.sum() < x.sum()      # counting starts where it is turned on
+
+
 def test_run_twice_identical():
     spec = netspec.standard(npop=4, n=500, fanout=80, w0=3.0, drive_rate_hz=400.0, n_drive=200,
                             drive_w=12.0, seed=3, i_offset=(11.0, 14.0))

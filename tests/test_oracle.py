@@ -198,3 +198,65 @@ def test_bernoulli_hash_known_values():
     z = ((z ^ (z >> 27)) * 0x94D049BB133111EB) & ((1 << 64) - 1)
     z ^= z >> 31
     assert vals[0] == z >> 32
+
+
+# ---- FireInfo logs "X" and "Y" (the GPU simulator's per-neuron counters) ----
+
+def _slot_count_closed_form(spec, raster_count, raster_ids, steps):
+    """X[n] = sum over n's spikes at cycle s of min(D, T - 1 - s): the
+    deliveries of cycles s+1 .. s+D each visit the spike once
+    (src/synapse/static/static.cu:42), D the network's largest delay."""
+    _, ds = oracle.synapse_reset(spec, DT)
+    D = int(ds.max()) if len(ds) else 0
+    cyc = np.repeat(np.arange(steps), raster_count)
+    x = np.zeros(spec.n, np.int64)
+    np.add.at(x, raster_ids, np.minimum(D, steps - 1 - cyc))
+    return x
+
+
+@pytest.mark.parametrize("seed", range(4))
+@pytest.mark.parametrize("mode", ["exact", "csr"])
+def test_slot_count_closed_form(seed, mode):
+    spec = netspec.random_net(seed, n_lif=50, n_arr=6, n_bern=4, p_conn=0.2, dmax=2 + 3 * seed,
+                              spike_prob=0.1, steps=150)
+    steps = 150
+    o = oracle.run(spec, DT, steps, mode=mode, tile=64, threads=2)
+    want = _slot_count_closed_form(spec, o.raster_count, o.raster_ids, steps)
+    assert o.raster_ids.size > 50
+    np.testing.assert_array_equal(o.slot_count, want)
+
+
+def test_input_sum_closed_form():
+    """Y (lif.cu:116) on targets that never fire: the sum over cycles of the
+    cycle's E + I input, i.e. every weight delivered at a cycle c <= T - 2
+    (consumed by the update of c + 1).  Checked against a float64 sum of the
+    deliveries computed from the sources' spike lists."""
+    rng = np.random.default_rng(3)
+    steps = 120
+    b = netspec.Builder()
+    spikes = [np.sort(rng.choice(steps, size=rng.integers(5, 40), replace=False)) for _ in range(8)]
+    src = b.population(np.zeros((8, netspec.LIF_NPARAM)), kind=netspec.ARRAY, spikes=spikes)
+    tgt = b.population(np.tile(netspec.lif_row(0.0, 0.0, 0.0, 0.02, 0.02, 2e-3, 5e-3, 8e-3, 1e9, 0.0),
+                               (5, 1)))
+    pre, post, w, d = [], [], [], []
+    for p in range(8):
+        for q in range(5):
+            if rng.random() < 0.7:
+                pre.append(p)
+                post.append(8 + q)
+                w.append(rng.uniform(-1.0, 1.0))
+                d.append(int(rng.integers(1, 6)) * DT)
+    b.connect(pre, post, np.asarray(w, np.float32), np.asarray(d, np.float32))
+    spec = b.build()
+    o = oracle.run(spec, DT, steps)
+    assert o.raster_count[:].sum() == sum(len(s) for s in spikes)   # only the sources fire
+    want = np.zeros(spec.n)
+    for k in range(len(pre)):
+        ds = int(round(float(np.float32(d[k])) / DT))
+        for s in spikes[pre[k]]:
+            if s + ds + 1 <= steps - 1:
+                want[post[k]] += float(np.float32(w[k]))
+    np.testing.assert_allclose(o.input_sum[8:].astype(np.float64), want[8:], rtol=1e-5, atol=1e-5)
+    assert np.all(o.input_sum[:8] == 0)
+    oc = oracle.run(spec, DT, steps, mode="csr", tile=64, threads=2)
+    assert np.array_equal(oc.input_sum.view(np.uint32), o.input_sum.view(np.uint32))
