@@ -211,6 +211,23 @@ struct Acc {
     float e[S_], i[S_];
 };
 
+template <int S_>
+__device__ __forceinline__ void add_event(Acc<S_> &acc, uint2 ev)
+{
+    const float x = __uint_as_float(ev.y);
+    const uint32_t slot = ev.x >> 6;
+    const bool exc = x >= 0.0f;
+#pragma unroll
+    for (int j = 0; j < S_; j++) {
+        if (slot == (uint32_t)j) {
+            if (exc)
+                acc.e[j] = acc.e[j] + x;
+            else
+                acc.i[j] = acc.i[j] + x;
+        }
+    }
+}
+
 /* Applies one streaming step: every lane applies, in stream order, the events
  * whose target it owns.  For each group, the event in lane i sets bit i of its
  * owner's mask with an LDS atomic OR (order-free); the owner walks its mask
@@ -240,20 +257,16 @@ __device__ __forceinline__ void apply_step(const Shared &S, const Step &st, uint
     for (int g = 0; g < NGRP; g++) {
         uint64_t m = M[g];
         while (m) {
-            const uint2 ev = stg[g * WAVE + (uint32_t)__builtin_ctzll(m)];
+            /* up to 2 owned events per round: their staged reads are issued
+             * together, the adds stay in stream order */
+            const uint32_t i0 = (uint32_t)__builtin_ctzll(m);
             m &= m - 1;
-            const float x = __uint_as_float(ev.y);
-            const uint32_t slot = ev.x >> 6;
-            const bool exc = x >= 0.0f;
-#pragma unroll
-            for (int j = 0; j < S_; j++) {
-                if (slot == (uint32_t)j) {
-                    if (exc)
-                        acc.e[j] = acc.e[j] + x;
-                    else
-                        acc.i[j] = acc.i[j] + x;
-                }
-            }
+            uint32_t i1 = i0; bool h1 = m != 0ull;
+            if (h1) { i1 = (uint32_t)__builtin_ctzll(m); m &= m - 1; }
+            const uint2 e0 = stg[g * WAVE + i0];
+            const uint2 e1 = stg[g * WAVE + i1];
+            add_event<S_>(acc, e0);
+            if (h1) add_event<S_>(acc, e1);
         }
     }
     wave_sync();
