@@ -71,7 +71,16 @@ def host_info():
                 break
     except OSError:
         pass
-    return dict(cpu=model, nproc=os.cpu_count(), machine=platform.machine())
+    mem_gb = None
+    try:
+        for line in open("/proc/meminfo"):
+            if line.startswith("MemTotal:"):
+                mem_gb = round(int(line.split()[1]) / 2 ** 20, 1)
+                break
+    except OSError:
+        pass
+    return dict(cpu=model, nproc=os.cpu_count(), machine=platform.machine(), mem_gib=mem_gb,
+                affinity=len(os.sched_getaffinity(0)) if hasattr(os, "sched_getaffinity") else None)
 
 
 def make_workload(name):
