@@ -40,6 +40,7 @@ struct bsim_sim {
     hipGraphExec_t gexec = nullptr;
     int32_t rec_steps = 0, n_rec = 0;
     int64_t n_src = 0, n_off = 0, s = 0;
+    std::vector<int32_t> rec_col;  /* trace column of each requested id (ids may repeat) */
     float *log_in = nullptr;       /* input logs (bsim_sim_enable_input_log) */
     int32_t *log_slot = nullptr;
 };
@@ -412,18 +413,25 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
             if (!(d.raster = dev_alloc<uint64_t>(sim, rb, e))) goto hip_fail;
             hipMemsetAsync(d.raster, 0, rb * 8, sim->stream);
             if (c.n_rec > 0) {
+                /* one device column per distinct neuron; repeated ids share it */
                 std::vector<int32_t> idx(L.n, -1);
-                for (int32_t r = 0; r < c.n_rec; r++) idx[c.rec_ids[r]] = r;
+                int32_t ncol = 0;
+                sim->rec_col.resize((size_t)c.n_rec);
+                for (int32_t r = 0; r < c.n_rec; r++) {
+                    int32_t &x = idx[c.rec_ids[r]];
+                    if (x < 0) x = ncol++;
+                    sim->rec_col[r] = x;
+                }
                 d.rec_index = dev_upload(sim, idx, e);
                 if (e != hipSuccess) goto hip_fail;
-                d.n_rec = c.n_rec;
-                const size_t vb = (size_t)c.rec_steps * (size_t)c.n_rec;
+                d.n_rec = ncol;
+                const size_t vb = (size_t)c.rec_steps * (size_t)ncol;
                 if (!(d.rec_v = dev_alloc<float>(sim, vb, e))) goto hip_fail;
                 hipMemsetAsync(d.rec_v, 0, vb * 4, sim->stream);
             }
         }
         sim->rec_steps = c.rec_steps;
-        sim->n_rec = d.n_rec;
+        sim->n_rec = c.rec_steps > 0 ? c.n_rec : 0;
         e = hipStreamSynchronize(sim->stream);
         if (e != hipSuccess) goto hip_fail;
     }
@@ -604,8 +612,13 @@ int32_t bsim_sim_read_trace(bsim_sim *sim, float *v)
     if (!sim->d.rec_v) return fail(BSIM_E_STATE, "no neuron is traced");
     hipSetDevice(sim->device);
     const int32_t steps = sim->cycle < sim->rec_steps ? sim->cycle : sim->rec_steps;
-    hipError_t e = fetch(sim, v, sim->d.rec_v, (size_t)steps * sim->n_rec);
+    const int32_t ncol = sim->d.n_rec;
+    std::vector<float> cols((size_t)steps * (size_t)ncol);
+    hipError_t e = fetch(sim, cols.data(), sim->d.rec_v, cols.size());
     if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("trace copy: ") + hipGetErrorString(e));
+    for (int32_t t = 0; t < steps; t++)
+        for (int32_t r = 0; r < sim->n_rec; r++)
+            v[(size_t)t * sim->n_rec + r] = cols[(size_t)t * ncol + sim->rec_col[r]];
     return steps;
 }
 

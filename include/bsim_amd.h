@@ -111,7 +111,7 @@ typedef struct bsim_sim_config {
     int32_t tile;          /* neurons per tile (multiple of 192, at most 1536); 0 = automatic:
                               192 for delays up to 8 steps, 384 up to 24, else 768 */
     int32_t rec_steps;     /* cycles of raster/trace kept on the device (0 = none) */
-    int32_t n_rec;         /* neurons whose V is traced */
+    int32_t n_rec;         /* neurons whose V is traced (ids may repeat: one column each) */
     const int32_t *rec_ids;
 } bsim_sim_config;
 

@@ -258,3 +258,81 @@ def test_dense_overflow_paths(tile):
     o = oracle.run(spec, DT, steps, rec_ids=rec, mode="csr", tile=256, threads=8)
     assert o.events > steps * 100_000
     assert_same(r, o, "dense")
+
+
+def _edge_net(kind):
+    b = netspec.Builder()
+    lif = netspec.lif_row(0.0, 0.0, 0.0, 0.02, 0.02, 2e-3, 5e-3, 8e-3, 1.0, 0.0)
+    if kind == "no_synapses":
+        rows = np.tile(lif, (300, 1))
+        rows[:, 9] = np.linspace(0.5, 4.0, 300)          # tonic firing above 1.0, no input
+        b.population(rows)
+    elif kind == "one_neuron":
+        r = lif.copy()
+        r[9] = 5.0
+        b.population(r[None, :])
+        b.connect([0], [0], np.float32([0.2]), np.float32([3 * DT]))    # self-synapse
+    elif kind == "sources_only":
+        rng = np.random.default_rng(5)
+        sp = [np.sort(rng.choice(200, size=20, replace=False)) for _ in range(7)]
+        b.population(np.zeros((7, netspec.LIF_NPARAM)), kind=netspec.ARRAY, spikes=sp)
+        b.population(np.zeros((5, netspec.LIF_NPARAM)), kind=netspec.BERNOULLI,
+                     bern_p=np.uint32(int(0.1 * 2 ** 32)))
+        pre = np.repeat(np.arange(12), 12)
+        post = np.tile(np.arange(12), 12)
+        b.connect(pre, post, np.float32(np.linspace(-1, 1, 144)), np.float32(np.full(144, 2 * DT)))
+    elif kind == "all_inhibitory":
+        rows = np.tile(lif, (250, 1))
+        rows[:50, 9] = np.linspace(2.0, 4.0, 50)
+        rows[50:, 9] = np.linspace(1.0, 1.8, 200)
+        b.population(rows[:50])
+        b.population(rows[50:])
+        rng = np.random.default_rng(6)
+        pre = np.repeat(np.arange(50), 40)
+        post = 50 + np.concatenate([np.sort(rng.choice(200, 40, replace=False)) for _ in range(50)])
+        b.connect(pre, post, np.float32(-rng.uniform(0, 0.3, len(pre))),
+                  np.float32(rng.integers(1, 5, len(pre)) * DT))
+    return b.build()
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("kind", ["no_synapses", "one_neuron", "sources_only", "all_inhibitory"])
+@pytest.mark.parametrize("tile", [0, 1536])
+def test_edge_networks_vs_oracle(kind, tile):
+    """Networks without synapses, of one neuron (with a self-synapse), of
+    sources only, and with only inhibitory synapses; a tile much larger than
+    the network: raster, V, state and the input logs bit-exact."""
+    spec = _edge_net(kind)
+    steps = 200
+    rec = [0, spec.n - 1]
+    say(f"[gpu] edge {kind} n={spec.n} s={spec.s} tile={tile}")
+    net = api.Network.from_spec(spec)
+    sim = api.Simulation(net, DT, tile=tile, rec_steps=steps, rec_ids=rec)
+    sim.enable_input_log()
+    sim.step(steps)
+    sim.finish()
+    sim.sync()
+    counts, ids, _ = sim.raster()
+    r = dict(raster_count=counts, raster_ids=ids, rec_v=sim.trace(), events=sim.events(),
+             info=sim.info(), **sim.state())
+    y, x = sim.input_log()
+    sim.close()
+    o = oracle.run(spec, DT, steps, rec_ids=rec)
+    assert_same(r, o, kind)
+    assert np.array_equal(y.view(np.uint32), o.input_sum.view(np.uint32)), kind
+    np.testing.assert_array_equal(x, o.slot_count, err_msg=kind)
+    if kind == "no_synapses":
+        assert r["events"] == 0 and x.sum() == 0 and len(ids) > 0
+
+
+@pytest.mark.gpu
+def test_zero_steps_and_finish():
+    spec = _edge_net("one_neuron")
+    sim = api.Simulation(api.Network.from_spec(spec), DT, rec_steps=10)
+    sim.step(0)
+    sim.finish()
+    sim.sync()
+    assert sim.events() == 0
+    counts, ids, _ = sim.raster()
+    assert len(counts) == 0 and len(ids) == 0
+    sim.close()
