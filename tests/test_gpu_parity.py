@@ -336,3 +336,49 @@ def test_zero_steps_and_finish():
     counts, ids, _ = sim.raster()
     assert len(counts) == 0 and len(ids) == 0
     sim.close()
+
+
+@pytest.mark.parametrize("dt", [5e-5, 2e-4])
+def test_other_time_steps_vs_oracle(dt):
+    """dt other than 0.1 ms: LIF constants, refractory steps and delay steps
+    all follow dt (delays are whole multiples of the new dt)."""
+    spec = netspec.random_net(77, n_lif=200, n_arr=10, n_bern=10, p_conn=0.15, dmax=7,
+                              spike_prob=0.05, steps=400)
+    spec.d = (np.rint(np.asarray(spec.d, np.float64) / DT) * dt).astype(np.float32)
+    steps = 400
+    net = api.Network.from_spec(spec)
+    sim = api.Simulation(net, dt, rec_steps=steps, rec_ids=[20, 150])
+    sim.step(steps)
+    sim.finish()
+    sim.sync()
+    counts, ids, _ = sim.raster()
+    r = dict(raster_count=counts, raster_ids=ids, rec_v=sim.trace(), events=sim.events(),
+             info=sim.info(), **sim.state())
+    sim.close()
+    o = oracle.run(spec, dt, steps, rec_ids=[20, 150])
+    assert len(ids) > 50
+    assert_same(r, o, f"dt={dt}")
+
+
+def test_two_simulations_interleaved():
+    """Simulations share no device state: two stepped in alternation give the
+    rasters of each run alone."""
+    a = netspec.standard(npop=3, n=400, fanout=60, w0=3.0, drive_rate_hz=400.0, n_drive=100,
+                         seed=31, i_offset=(12.0, 14.9))
+    b = netspec.random_net(32, n_lif=300, n_arr=10, n_bern=10, p_conn=0.1, dmax=9,
+                           spike_prob=0.05, steps=300)
+    steps = 300
+    alone = [run_gpu(s, steps, [1]) for s in (a, b)]
+    sims = [api.Simulation(api.Network.from_spec(s), DT, rec_steps=steps, rec_ids=[1])
+            for s in (a, b)]
+    for c in range(0, steps, 50):
+        for sim in sims:
+            sim.step(50)
+    for sim, ref in zip(sims, alone):
+        sim.finish()
+        sim.sync()
+        counts, ids, _ = sim.raster()
+        np.testing.assert_array_equal(counts, ref["raster_count"])
+        np.testing.assert_array_equal(ids, ref["raster_ids"])
+        assert sim.events() == ref["events"]
+        sim.close()
