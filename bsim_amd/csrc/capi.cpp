@@ -349,8 +349,18 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
     UP(tile_nsyn, L.tile_nsyn);
     UP(tile_res, L.tile_res);
     d.nslots = L.nslots;
-    UP(syn_tgt, L.syn_tgt);
-    UP(syn_w, L.syn_w);
+    {
+        /* targets and weights packed into one 8-byte word per synapse */
+        std::vector<bsim_syn> packed(L.syn_tgt.size());
+#pragma omp parallel for schedule(static)
+        for (int64_t i = 0; i < (int64_t)packed.size(); i++) {
+            packed[i].tgt = L.syn_tgt[i];
+            memcpy(&packed[i].w, &L.syn_w[i], 4);
+        }
+        std::vector<uint16_t>().swap(L.syn_tgt);
+        std::vector<float>().swap(L.syn_w);
+        UP(syn, packed);
+    }
     UP(tile_rng, L.tile_rng);
     UP(rng, L.rng);
     UP(off_tab, L.off_tab);

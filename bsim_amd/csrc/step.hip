@@ -171,7 +171,7 @@ struct Step {
  * and the apply of the previous step waits only for its own loads. */
 __device__ __forceinline__ void load_step(const Shared &S, const uint32_t *rows, RowWin &W, uint32_t b,
                                           uint32_t tot, int32_t nrows, uint32_t nsyn,
-                                          const uint16_t *tgt, const float *wts, Step &st)
+                                          const bsim_syn *syn, Step &st)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     const uint32_t q = threadIdx.x / WAVE;
@@ -198,8 +198,9 @@ __device__ __forceinline__ void load_step(const Shared &S, const uint32_t *rows,
         uint32_t addr = v ? al[pr] : 0u;
         addr = (addr < nsyn) ? addr : 0u;
         st.v[g] = v;
-        st.tl[g] = tgt[addr];
-        st.w[g] = wts[addr];
+        const bsim_syn sv = syn[addr];
+        st.tl[g] = sv.tgt;
+        st.w[g] = __uint_as_float(sv.w);
     }
     wave_sync();
 }
@@ -276,21 +277,21 @@ __device__ __forceinline__ void apply_step(const Shared &S, const Step &st, uint
  * of step s+1 are in flight while step s is applied. */
 template <int S_>
 __device__ void stream_rows(const Shared &S, const uint32_t *rows, int32_t nrows, uint32_t tot,
-                            uint32_t qbase, uint32_t nsyn, const uint16_t *tgt, const float *wts,
+                            uint32_t qbase, uint32_t nsyn, const bsim_syn *syn,
                             Acc<S_> &acc)
 {
     if (tot == 0) return;                                     /* uniform */
     RowWin W;
     load_win(rows, W, 0, 0, nrows);
     Step A, B;
-    load_step(S, rows, W, 0, tot, nrows, nsyn, tgt, wts, A);
+    load_step(S, rows, W, 0, tot, nrows, nsyn, syn, A);
     uint32_t b = 0;
     while (true) {
-        load_step(S, rows, W, b + SSTEP, tot, nrows, nsyn, tgt, wts, B);
+        load_step(S, rows, W, b + SSTEP, tot, nrows, nsyn, syn, B);
         apply_step<S_>(S, A, qbase, acc);
         b += SSTEP;
         if (b >= tot) break;
-        load_step(S, rows, W, b + SSTEP, tot, nrows, nsyn, tgt, wts, A);
+        load_step(S, rows, W, b + SSTEP, tot, nrows, nsyn, syn, A);
         apply_step<S_>(S, B, qbase, acc);
         b += SSTEP;
         if (b >= tot) break;
@@ -375,7 +376,7 @@ __device__ __forceinline__ void prod_stage(const Shared &S, const bsim_dev &d, i
  * wavefronts 0..NWAVE-1 stream window i. */
 template <int S_>
 __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, int32_t nrb,
-                              uint32_t qbase, uint32_t nsyn, const uint16_t *tgt, const float *wts,
+                              uint32_t qbase, uint32_t nsyn, const bsim_syn *syn,
                               int64_t &events, Acc<S_> &acc)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
@@ -431,7 +432,7 @@ __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, in
             const int32_t nrows = (int32_t)min((uint32_t)RCAP, nrows_tot - (uint32_t)i * RCAP);
             const uint32_t qtot = S.tot[buf * NWAVE + w];
             events += qtot;
-            stream_rows<S_>(S, S.rows + buf * (RCAP * ROWW), nrows, qtot, qbase, nsyn, tgt, wts, acc);
+            stream_rows<S_>(S, S.rows + buf * (RCAP * ROWW), nrows, qtot, qbase, nsyn, syn, acc);
         }
         __syncthreads();
     }
@@ -487,12 +488,11 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
     if ((flags & 1) && k >= 0) {
         const int64_t syn0 = d.tile_base[k];
         const uint32_t nsyn = d.tile_nsyn[k];
-        const uint16_t *tgt = d.syn_tgt + syn0;
-        const float *wts = d.syn_w + syn0;
+        const bsim_syn *syn = d.syn + syn0;
         int64_t events = 0;
         const int32_t r0 = d.tile_rng[k], r1 = d.tile_rng[k + 1];
         for (int32_t rb = r0; rb < r1; rb += MAXR)
-            deliver_batch<S_>(S, d, rb, min(MAXR, r1 - rb), qbase, nsyn, tgt, wts, events, acc);
+            deliver_batch<S_>(S, d, rb, min(MAXR, r1 - rb), qbase, nsyn, syn, events, acc);
         if (lane == 0 && w < NWAVE)
             atomicAdd((unsigned long long *)&d.ev_count[k], (unsigned long long)events);
     }
