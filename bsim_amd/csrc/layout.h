@@ -16,7 +16,8 @@
  * sorted by (source range, pre, delay_steps, tile, part of the target,
  * creation index) -- the (pre, delay) CSR of the reference
  * (src/net/BuildNetwork.cpp:96-176) restricted to the group, each row split
- * by tile and part -- as a 16-bit local target and an fp32 weight.  Sorting
+ * by tile and part -- as a tile-local target and an fp32 weight packed in
+ * one 8-byte word (bsim_syn).  Sorting
  * inside a row by tile and part keeps, for every target, the (pre, delay,
  * creation) order of its inputs; keeping a row's pieces for all tiles of the
  * group adjacent lets those tiles share cache lines.  Per group and source
@@ -59,12 +60,18 @@ typedef struct bsim_range {
     uint32_t pad;
 } bsim_range;
 
-/* One synapse on the device: the tile-local target and the fp32 weight bits
- * in one aligned 8-byte word, so that an event is one load of one cache line. */
+/* One synapse on the device, for tiles up to BSIM_PACKED_MAX_TILE: the
+ * tile-local target and the fp32 weight bits in one aligned 8-byte word, so
+ * that an event is one load of one cache line.  Larger tiles (deep delay
+ * rings, dense activity, where a launch reads long stretches of the synapse
+ * block) keep the 2-byte targets and 4-byte weights in two arrays: fewer
+ * bytes per synapse. */
 typedef struct __attribute__((aligned(8))) bsim_syn {
     uint32_t tgt;
     uint32_t w;
 } bsim_syn;
+
+#define BSIM_PACKED_MAX_TILE (2 * BSIM_TILE_ALIGN)
 
 #define BSIM_MAX_RANGE (1 << 26)   /* pres per source range: row index fits 32 bits */
 #define BSIM_CHUNK 4096             /* pres per source chunk (one row-list workgroup) */
@@ -99,7 +106,10 @@ typedef struct bsim_dev {
     const int64_t *tile_base;   /* [ntiles] first synapse of the tile's group */
     const uint32_t *tile_nsyn;  /* [ntiles] synapses of the tile's group */
     const uint8_t *tile_res;    /* [ntiles] XCD residue of the tile's group */
-    const bsim_syn *syn;        /* [S] target and weight of each synapse, group-major */
+    const bsim_syn *syn;        /* [S] target and weight of each synapse, group-major
+                                   (tile <= BSIM_PACKED_MAX_TILE; else NULL) */
+    const uint16_t *syn_tgt;    /* [S] the same as two arrays (larger tiles; else NULL) */
+    const float *syn_w;
     const int32_t *tile_rng;    /* [ntiles + 1] first source range of each tile */
     const bsim_range *rng;
     const uint32_t *off_tab;    /* group row tables (16-byte aligned vectors) */
