@@ -349,7 +349,7 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
     UP(tile_nsyn, L.tile_nsyn);
     UP(tile_res, L.tile_res);
     d.nslots = L.nslots;
-    if (L.tile <= BSIM_PACKED_MAX_TILE) {
+    {
         /* targets and weights packed into one 8-byte word per synapse */
         std::vector<bsim_syn> packed(L.syn_tgt.size());
 #pragma omp parallel for schedule(static)
@@ -360,9 +360,6 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
         std::vector<uint16_t>().swap(L.syn_tgt);
         std::vector<float>().swap(L.syn_w);
         UP(syn, packed);
-    } else {
-        UP(syn_tgt, L.syn_tgt);
-        UP(syn_w, L.syn_w);
     }
     UP(tile_rng, L.tile_rng);
     UP(rng, L.rng);

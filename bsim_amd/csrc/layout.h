@@ -60,18 +60,14 @@ typedef struct bsim_range {
     uint32_t pad;
 } bsim_range;
 
-/* One synapse on the device, for tiles up to BSIM_PACKED_MAX_TILE: the
- * tile-local target and the fp32 weight bits in one aligned 8-byte word, so
- * that an event is one load of one cache line.  Larger tiles (deep delay
- * rings, dense activity, where a launch reads long stretches of the synapse
- * block) keep the 2-byte targets and 4-byte weights in two arrays: fewer
- * bytes per synapse. */
+/* One synapse on the device: the tile-local target and the fp32 weight bits
+ * in one aligned 8-byte word, so that an event is one load of one cache line
+ * (with separate 2-byte target and 4-byte weight arrays an event was two
+ * loads touching two lines). */
 typedef struct __attribute__((aligned(8))) bsim_syn {
     uint32_t tgt;
     uint32_t w;
 } bsim_syn;
-
-#define BSIM_PACKED_MAX_TILE (2 * BSIM_TILE_ALIGN)
 
 #define BSIM_MAX_RANGE (1 << 26)   /* pres per source range: row index fits 32 bits */
 #define BSIM_CHUNK 4096             /* pres per source chunk (one row-list workgroup) */
@@ -106,10 +102,7 @@ typedef struct bsim_dev {
     const int64_t *tile_base;   /* [ntiles] first synapse of the tile's group */
     const uint32_t *tile_nsyn;  /* [ntiles] synapses of the tile's group */
     const uint8_t *tile_res;    /* [ntiles] XCD residue of the tile's group */
-    const bsim_syn *syn;        /* [S] target and weight of each synapse, group-major
-                                   (tile <= BSIM_PACKED_MAX_TILE; else NULL) */
-    const uint16_t *syn_tgt;    /* [S] the same as two arrays (larger tiles; else NULL) */
-    const float *syn_w;
+    const bsim_syn *syn;        /* [S] target and weight of each synapse, group-major */
     const int32_t *tile_rng;    /* [ntiles + 1] first source range of each tile */
     const bsim_range *rng;
     const uint32_t *off_tab;    /* group row tables (16-byte aligned vectors) */

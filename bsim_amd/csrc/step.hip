@@ -155,13 +155,6 @@ __device__ __forceinline__ void load_win(const uint32_t *rows, RowWin &W, int32_
     W.end = pos0 + readlane(incl, WAVE - 1);
 }
 
-/* The synapses of a tile's group: packed words, or target and weight arrays. */
-struct SynSrc {
-    const bsim_syn *pk;
-    const uint16_t *tgt;
-    const float *w;
-};
-
 /* Loaded events of one streaming step: NGRP groups of 64. */
 struct Step {
     uint32_t tl[NGRP];
@@ -176,10 +169,9 @@ struct Step {
  * b + 64 g + i.  The loads are issued unconditionally (lanes without an event
  * read the tile's first synapse), so that the compiler can count them exactly
  * and the apply of the previous step waits only for its own loads. */
-template <bool PK>
 __device__ __forceinline__ void load_step(const Shared &S, const uint32_t *rows, RowWin &W, uint32_t b,
                                           uint32_t tot, int32_t nrows, uint32_t nsyn,
-                                          const SynSrc &syn, Step &st)
+                                          const bsim_syn *syn, Step &st)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     const uint32_t q = threadIdx.x / WAVE;
@@ -206,14 +198,9 @@ __device__ __forceinline__ void load_step(const Shared &S, const uint32_t *rows,
         uint32_t addr = v ? al[pr] : 0u;
         addr = (addr < nsyn) ? addr : 0u;
         st.v[g] = v;
-        if (PK) {
-            const bsim_syn sv = syn.pk[addr];
-            st.tl[g] = sv.tgt;
-            st.w[g] = __uint_as_float(sv.w);
-        } else {
-            st.tl[g] = syn.tgt[addr];
-            st.w[g] = syn.w[addr];
-        }
+        const bsim_syn sv = syn[addr];
+        st.tl[g] = sv.tgt;
+        st.w[g] = __uint_as_float(sv.w);
     }
     wave_sync();
 }
@@ -291,21 +278,21 @@ __device__ __forceinline__ void apply_step(const Shared &S, const Step &st, uint
  * of step s+1 are in flight while step s is applied. */
 template <int S_>
 __device__ void stream_rows(const Shared &S, const uint32_t *rows, int32_t nrows, uint32_t tot,
-                            uint32_t qbase, uint32_t nsyn, const SynSrc &syn,
+                            uint32_t qbase, uint32_t nsyn, const bsim_syn *syn,
                             Acc<S_> &acc)
 {
     if (tot == 0) return;                                     /* uniform */
     RowWin W;
     load_win(rows, W, 0, 0, nrows);
     Step A, B;
-    load_step<(S_ <= BSIM_PACKED_MAX_TILE / BSIM_TILE_ALIGN)>(S, rows, W, 0, tot, nrows, nsyn, syn, A);
+    load_step(S, rows, W, 0, tot, nrows, nsyn, syn, A);
     uint32_t b = 0;
     while (true) {
-        load_step<(S_ <= BSIM_PACKED_MAX_TILE / BSIM_TILE_ALIGN)>(S, rows, W, b + SSTEP, tot, nrows, nsyn, syn, B);
+        load_step(S, rows, W, b + SSTEP, tot, nrows, nsyn, syn, B);
         apply_step<S_>(S, A, qbase, acc);
         b += SSTEP;
         if (b >= tot) break;
-        load_step<(S_ <= BSIM_PACKED_MAX_TILE / BSIM_TILE_ALIGN)>(S, rows, W, b + SSTEP, tot, nrows, nsyn, syn, A);
+        load_step(S, rows, W, b + SSTEP, tot, nrows, nsyn, syn, A);
         apply_step<S_>(S, B, qbase, acc);
         b += SSTEP;
         if (b >= tot) break;
@@ -390,7 +377,7 @@ __device__ __forceinline__ void prod_stage(const Shared &S, const bsim_dev &d, i
  * wavefronts 0..NWAVE-1 stream window i. */
 template <int S_>
 __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, int32_t nrb,
-                              uint32_t qbase, uint32_t nsyn, const SynSrc &syn,
+                              uint32_t qbase, uint32_t nsyn, const bsim_syn *syn,
                               int64_t &events, Acc<S_> &acc)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
@@ -502,8 +489,7 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
     if ((flags & 1) && k >= 0) {
         const int64_t syn0 = d.tile_base[k];
         const uint32_t nsyn = d.tile_nsyn[k];
-        const SynSrc syn = {d.syn ? d.syn + syn0 : nullptr, d.syn_tgt ? d.syn_tgt + syn0 : nullptr,
-                            d.syn_w ? d.syn_w + syn0 : nullptr};
+        const bsim_syn *syn = d.syn + syn0;
         int64_t events = 0;
         const int32_t r0 = d.tile_rng[k], r1 = d.tile_rng[k + 1];
         for (int32_t rb = r0; rb < r1; rb += MAXR)
