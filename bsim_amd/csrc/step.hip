@@ -212,18 +212,19 @@ struct Acc {
     float e[S_], i[S_];
 };
 
-/* Adds one owned event to the accumulator of its slot (E if w >= 0, else I).
- * Written as selects over statically indexed accumulators, so that they stay
- * in registers for every S_. */
+/* Adds one owned event to the accumulator of its slot (E if w >= 0, else I)
+ * when `valid`.  Written as selects over statically indexed accumulators, so
+ * that they stay in registers for every S_ and no branch separates the staged
+ * reads of a round. */
 template <int S_>
-__device__ __forceinline__ void add_event(Acc<S_> &acc, uint2 ev)
+__device__ __forceinline__ void add_event(Acc<S_> &acc, uint2 ev, bool valid)
 {
     const float x = __uint_as_float(ev.y);
     const uint32_t slot = ev.x >> 6;
     const bool exc = x >= 0.0f;
 #pragma unroll
     for (int j = 0; j < S_; j++) {
-        const bool mine = slot == (uint32_t)j;
+        const bool mine = valid && slot == (uint32_t)j;
         const float ae = acc.e[j] + x, ai = acc.i[j] + x;
         acc.e[j] = (mine && exc) ? ae : acc.e[j];
         acc.i[j] = (mine && !exc) ? ai : acc.i[j];
@@ -263,12 +264,13 @@ __device__ __forceinline__ void apply_step(const Shared &S, const Step &st, uint
              * together, the adds stay in stream order */
             const uint32_t i0 = (uint32_t)__builtin_ctzll(m);
             m &= m - 1;
-            uint32_t i1 = i0; bool h1 = m != 0ull;
+            uint32_t i1 = i0;
+            const bool h1 = m != 0ull;
             if (h1) { i1 = (uint32_t)__builtin_ctzll(m); m &= m - 1; }
             const uint2 e0 = stg[g * WAVE + i0];
             const uint2 e1 = stg[g * WAVE + i1];
-            add_event<S_>(acc, e0);
-            if (h1) add_event<S_>(acc, e1);
+            add_event<S_>(acc, e0, true);
+            add_event<S_>(acc, e1, h1);
         }
     }
     wave_sync();
