@@ -27,9 +27,11 @@
  *           part of each: the concatenated sub-rows form a stream of
  *           positions; per step of 256 positions the sub-rows (64 rows at a
  *           time in the lanes) write the synapse index of each of their
- *           positions into an LDS address list, and lane i of group g loads
- *           position 64 g + i; the loads of step s+1 are in flight while step
- *           s is applied;
+ *           positions into an LDS address list -- from TILE 384 up, 128 rows
+ *           at a time write only their first position in the step and a bit
+ *           of a position mask, from which each position finds its sub-row
+ *           -- and lane i of group g loads position 64 g + i; the loads of
+ *           step s+1 are in flight while step s is applied;
  *   apply   the owner of local target x is lane x mod 64 of wavefront q, which
  *           keeps x's E and I accumulators in registers.  For each 64-event
  *           group, the event in lane i sets bit i of its owner's mask by an
@@ -64,9 +66,14 @@
 #define HEAVY_WGS 256                 /* workgroups run at raised wave priority */
 static_assert(BSIM_CHUNK == 4 * ROWS_THREADS, "a source chunk is one row-list workgroup");
 
-/* LDS bytes of a workgroup: lbase, obase | stage | rows | tot | alist | cum | stride | omask */
+#ifndef MARK_MIN_S
+#define MARK_MIN_S 2                  /* slots per lane from which sub-rows are expanded by marks */
+#endif
+static_assert(ROWW == 4 && NWAVE == 3, "a staged row is one 16-byte vector of 3 part starts and the end");
+
+/* LDS bytes of a workgroup: lbase, obase | stage | rows | tot | alist | cum | stride | omask | pmask */
 #define LDS_FIXED (16 * MAXR + 8 * NWAVE * SSTEP + 4 * (2 * RCAP * ROWW + 2 * NWAVE + NWAVE * SSTEP) + \
-                   4 * (MAXR + 2) + 4 * MAXR + 8 * NWAVE * NGRP * WAVE)
+                   4 * (MAXR + 2) + 4 * MAXR + 8 * NWAVE * NGRP * WAVE + 8 * NWAVE * NGRP)
 
 __device__ __forceinline__ uint32_t bern_hash(uint64_t seed, uint32_t id, uint32_t t)
 {
@@ -119,6 +126,7 @@ struct Shared {
     uint32_t *tot;       /* [2][NWAVE] stream length of each quarter in the window */
     uint2 *stage;        /* [NWAVE][SSTEP] (local target in quarter, weight bits) */
     uint64_t *omask;     /* [NWAVE][NGRP][WAVE] events of a group owned by each lane */
+    uint64_t *pmask;     /* [NWAVE][NGRP] step positions where a sub-row starts (mark expansion) */
     uint32_t *alist;     /* [NWAVE][SSTEP] synapse index of each position of a step */
     uint32_t *cum;       /* [MAXR + 1] rows before each source range of the batch */
     uint32_t *stride;    /* [MAXR] row-table entries per row of each source range */
@@ -126,32 +134,51 @@ struct Shared {
     int64_t *obase;      /* [MAXR] row-table base of each source range */
 };
 
-/* A window of 64 staged rows in the lanes of a wavefront: lane j holds row
- * rw + j of the wavefront's sub-row stream. */
+/* A window of 64 R staged rows in the lanes of a wavefront: lane j holds
+ * rows rw + R j .. rw + R j + R - 1 of the wavefront's sub-row stream. */
+template <int R>
 struct RowWin {
-    int32_t rw;      /* first row (uniform) */
-    uint32_t pos;    /* stream position of the sub-row */
-    uint32_t len;    /* sub-row length */
-    uint32_t base;   /* synapse index of the sub-row minus pos */
-    uint32_t end;    /* stream position after the window (uniform) */
+    int32_t rw;         /* first row (uniform) */
+    uint32_t pos[R];    /* stream position of each sub-row */
+    uint32_t len[R];    /* sub-row length */
+    uint32_t base[R];   /* synapse index of the sub-row minus pos */
+    uint32_t end;       /* stream position after the window (uniform) */
 };
 
-__device__ __forceinline__ void load_win(const uint32_t *rows, RowWin &W, int32_t rw, uint32_t pos0,
+template <int R>
+__device__ __forceinline__ void load_win(const uint32_t *rows, RowWin<R> &W, int32_t rw, uint32_t pos0,
                                          int32_t nrows)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     const uint32_t q = threadIdx.x / WAVE;
-    const int32_t j = rw + (int32_t)lane;
-    uint32_t s0 = 0, s1 = 0;
-    if (j < nrows) {
-        s0 = rows[j * ROWW + q];
-        s1 = rows[j * ROWW + q + 1];
+    uint32_t s0[R], sum = 0;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        const int32_t j = rw + R * (int32_t)lane + r;
+        uint32_t a = 0, c = 0;
+        if (j < nrows) {
+            if constexpr (R == 1) {
+                a = rows[j * ROWW + q];
+                c = rows[j * ROWW + q + 1];
+            } else {                      /* one 16-byte read per row */
+                const uint4 v = reinterpret_cast<const uint4 *>(rows)[j];
+                a = (q == 0) ? v.x : (q == 1) ? v.y : v.z;
+                c = (q == 0) ? v.y : (q == 1) ? v.z : v.w;
+            }
+        }
+        s0[r] = a;
+        W.len[r] = c - a;
+        sum += W.len[r];
+    }
+    const uint32_t incl = wave_incl_scan(sum);
+    uint32_t p = pos0 + incl - sum;
+#pragma unroll
+    for (int r = 0; r < R; r++) {
+        W.pos[r] = p;
+        W.base[r] = s0[r] - p;
+        p += W.len[r];
     }
     W.rw = rw;
-    W.len = s1 - s0;
-    const uint32_t incl = wave_incl_scan(W.len);
-    W.pos = pos0 + incl - W.len;
-    W.base = s0 - W.pos;
     W.end = pos0 + readlane(incl, WAVE - 1);
 }
 
@@ -162,40 +189,75 @@ struct Step {
     bool v[NGRP];
 };
 
-/* Loads the events of stream positions [b, b + SSTEP): the sub-rows of the
- * row window that intersect them write the synapse index of each of their
- * positions into the wavefront's address list (sliding the window over the
- * staged rows as needed), then lane i of group g loads position
- * b + 64 g + i.  The loads are issued unconditionally (lanes without an event
- * read the tile's first synapse), so that the compiler can count them exactly
- * and the apply of the previous step waits only for its own loads. */
-__device__ __forceinline__ void load_step(const Shared &S, const uint32_t *rows, RowWin &W, uint32_t b,
+/* Loads the events of stream positions [b, b + SSTEP) into lane i of group g
+ * (position b + 64 g + i), sliding the row window over the staged rows as
+ * needed.  Two ways to find each position's synapse index:
+ *   R == 1  every sub-row that intersects the step writes the index of each
+ *           of its positions into the wavefront's address list (one pass per
+ *           position of the window's longest sub-row);
+ *   R >= 2  (marks) every such sub-row writes only its first position in the
+ *           step: its synapse base into the address list and a bit into the
+ *           step's position mask; each position then takes the base of the
+ *           last mark at or before it (the highest set mask bit below it).
+ *           Costs one pass per window whatever the sub-row lengths.
+ * The loads are issued unconditionally (lanes without an event read the
+ * tile's first synapse), so that the compiler can count them exactly and the
+ * apply of the previous step waits only for its own loads. */
+template <int R>
+__device__ __forceinline__ void load_step(const Shared &S, const uint32_t *rows, RowWin<R> &W, uint32_t b,
                                           uint32_t tot, int32_t nrows, uint32_t nsyn,
                                           const bsim_syn *syn, Step &st)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     const uint32_t q = threadIdx.x / WAVE;
     uint32_t *al = S.alist + q * SSTEP;
+    uint64_t *pm = S.pmask + q * NGRP;
     const uint32_t e = min(b + SSTEP, tot);
     if (b < tot) {                                            /* uniform */
         while (true) {
-            const uint32_t lo = max(W.pos, b), hi = min(W.pos + W.len, e);
-            const uint32_t n = hi > lo ? hi - lo : 0u;
-            const uint32_t a0 = W.base + lo;
-            uint32_t *dst = al + (lo - b);
-            for (uint32_t x = 0; ballot64(x < n) != 0ull; x++)
-                if (x < n) dst[x] = a0 + x;
+#pragma unroll
+            for (int r = 0; r < R; r++) {
+                const uint32_t lo = max(W.pos[r], b), hi = min(W.pos[r] + W.len[r], e);
+                if constexpr (R == 1) {
+                    const uint32_t n = hi > lo ? hi - lo : 0u;
+                    const uint32_t a0 = W.base[r] + lo;
+                    uint32_t *dst = al + (lo - b);
+                    for (uint32_t x = 0; ballot64(x < n) != 0ull; x++)
+                        if (x < n) dst[x] = a0 + x;
+                } else if (hi > lo) {
+                    const uint32_t x = lo - b;
+                    al[x] = W.base[r];
+                    atomicOr((unsigned long long *)&pm[x / WAVE], 1ull << (x & (WAVE - 1)));
+                }
+            }
             if (W.end >= e) break;                            /* uniform */
-            if (W.rw + WAVE >= nrows) break;
-            load_win(rows, W, W.rw + WAVE, W.end, nrows);
+            if (W.rw + R * WAVE >= nrows) break;
+            load_win<R>(rows, W, W.rw + R * WAVE, W.end, nrows);
         }
     }
     wave_sync();
+    uint64_t M[NGRP];
+    if constexpr (R > 1) {
+#pragma unroll
+        for (int g = 0; g < NGRP; g++) M[g] = pm[g];
+        wave_sync();
+        if (lane < NGRP) pm[lane] = 0ull;
+    }
+    const uint64_t le = ~0ull >> (WAVE - 1 - lane);           /* bits 0..lane */
+    uint32_t last = 0;          /* the step's position 0 is always marked */
 #pragma unroll
     for (int g = 0; g < NGRP; g++) {
         const uint32_t pr = g * WAVE + lane;
         const bool v = b + pr < e;
-        uint32_t addr = v ? al[pr] : 0u;
+        uint32_t addr;
+        if constexpr (R == 1) {
+            addr = v ? al[pr] : 0u;
+        } else {
+            const uint64_t mg = M[g] & le;
+            const uint32_t idx = mg ? (uint32_t)(g * WAVE + WAVE - 1 - __clzll(mg)) : last;
+            last = M[g] ? (uint32_t)(g * WAVE + WAVE - 1 - __clzll(M[g])) : last;
+            addr = v ? al[idx] + b + pr : 0u;
+        }
         addr = (addr < nsyn) ? addr : 0u;
         st.v[g] = v;
         const bsim_syn sv = syn[addr];
@@ -284,8 +346,9 @@ __device__ void stream_rows(const Shared &S, const uint32_t *rows, int32_t nrows
                             Acc<S_> &acc)
 {
     if (tot == 0) return;                                     /* uniform */
-    RowWin W;
-    load_win(rows, W, 0, 0, nrows);
+    constexpr int R = (S_ >= MARK_MIN_S) ? 2 : 1;
+    RowWin<R> W;
+    load_win<R>(rows, W, 0, 0, nrows);
     Step A, B;
     load_step(S, rows, W, 0, tot, nrows, nsyn, syn, A);
     uint32_t b = 0;
@@ -474,6 +537,7 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
         S.cum = p;                                    p += MAXR + 2;
         S.stride = p;                                 p += MAXR;
         S.omask = reinterpret_cast<uint64_t *>(p);    /* 8-byte aligned: see LDS_FIXED */
+        S.pmask = S.omask + NWAVE * NGRP * WAVE;
     }
 
     Acc<S_> acc;
@@ -485,6 +549,7 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
     if (w < NWAVE) {
 #pragma unroll
         for (int g = 0; g < NGRP; g++) S.omask[(w * NGRP + g) * WAVE + lane] = 0ull;
+        if (lane < NGRP) S.pmask[w * NGRP + lane] = 0ull;
     }
 
     /* ---- phase (ii) of cycle t-1 ---------------------------------------- */
