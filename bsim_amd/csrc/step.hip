@@ -665,6 +665,7 @@ __device__ __forceinline__ uint32_t load_class(uint64_t x)
 
 #define RANK_LDS_TILES 16384   /* tile classes kept in LDS up to this many tiles */
 #define NRES 8                  /* XCDs: workgroup w runs on XCD (w + c) mod 8 */
+#define RANK_REG 4              /* tiles per thread kept in registers, up to RANK_REG * ROWS_THREADS tiles */
 
 __device__ void rank_tiles(const bsim_dev &d)
 {
@@ -673,30 +674,70 @@ __device__ void rank_tiles(const bsim_dev &d)
     const uint32_t tid = threadIdx.x;
     const bool in_lds = d.ntiles <= RANK_LDS_TILES;
     uint8_t *cls = in_lds ? cls_lds : d.tile_cls;
+    const bool in_regs = d.ntiles <= RANK_REG * ROWS_THREADS;
+    uint32_t xr[RANK_REG], cr[RANK_REG];
     for (uint32_t i = tid; i < NRES * RANK_CLASSES; i += ROWS_THREADS) (&cnt[0][0])[i] = 0u;
-    __syncthreads();
-    for (int32_t k = tid; k < d.ntiles; k += ROWS_THREADS) {
-        /* written by other workgroups' atomics in L2; this workgroup never read it */
-        const uint32_t c = load_class(d.tile_load[k]);
-        cls[k] = (uint8_t)c;
-        d.tile_load[k] = 0ull;
-        atomicAdd(&cnt[d.tile_res[k]][c], 1u);
+    if (in_regs) {              /* all loads of a thread issued together */
+        uint64_t ld[RANK_REG];
+#pragma unroll
+        for (int i = 0; i < RANK_REG; i++) {
+            const int32_t k = (int32_t)tid + i * ROWS_THREADS;
+            ld[i] = (k < d.ntiles) ? d.tile_load[k] : 0ull;
+            xr[i] = (k < d.ntiles) ? d.tile_res[k] : 0u;
+        }
+        __syncthreads();
+#pragma unroll
+        for (int i = 0; i < RANK_REG; i++) {
+            const int32_t k = (int32_t)tid + i * ROWS_THREADS;
+            cr[i] = load_class(ld[i]);
+            if (k < d.ntiles) {
+                d.tile_load[k] = 0ull;
+                atomicAdd(&cnt[xr[i]][cr[i]], 1u);
+            }
+        }
+    } else {
+        __syncthreads();
+        for (int32_t k = tid; k < d.ntiles; k += ROWS_THREADS) {
+            /* written by other workgroups' atomics in L2; this workgroup never read it */
+            const uint32_t c = load_class(d.tile_load[k]);
+            cls[k] = (uint8_t)c;
+            d.tile_load[k] = 0ull;
+            atomicAdd(&cnt[d.tile_res[k]][c], 1u);
+        }
     }
     __syncthreads();
-    if (tid < NRES) {
-        uint32_t s = 0;
-        for (int c = RANK_CLASSES - 1; c >= 0; c--) {   /* heaviest class first */
-            fill[tid][c] = s;
-            s += cnt[tid][c];
-        }
-        rtot[tid] = s;
+    /* exclusive prefix of each residue's class counts, heaviest class first:
+     * thread t takes residue t / RANK_CLASSES, class RANK_CLASSES - 1 - t mod
+     * RANK_CLASSES (two wavefronts per residue) */
+    static_assert(NRES * RANK_CLASSES == ROWS_THREADS && RANK_CLASSES == 2 * WAVE, "one thread per class");
+    {
+        __shared__ uint32_t half[ROWS_THREADS / WAVE];
+        const uint32_t x = tid / RANK_CLASSES, c = RANK_CLASSES - 1u - tid % RANK_CLASSES;
+        const uint32_t v = cnt[x][c];
+        const uint32_t incl = wave_incl_scan(v);
+        if ((tid & (WAVE - 1)) == WAVE - 1) half[tid / WAVE] = incl;
+        __syncthreads();
+        const uint32_t carry = ((tid / WAVE) & 1u) ? half[tid / WAVE - 1] : 0u;
+        fill[x][c] = carry + incl - v;
+        if (tid % RANK_CLASSES == RANK_CLASSES - 1u) rtot[x] = carry + incl;
     }
     __syncthreads();
     /* tile of residue x at rank i goes to workgroup NRES * i + x */
-    for (int32_t k = tid; k < d.ntiles; k += ROWS_THREADS) {
-        const uint32_t x = d.tile_res[k];
-        const uint32_t pos = atomicAdd(&fill[x][cls[k]], 1u);
-        d.perm[NRES * pos + x] = k;
+    if (in_regs) {
+#pragma unroll
+        for (int i = 0; i < RANK_REG; i++) {
+            const int32_t k = (int32_t)tid + i * ROWS_THREADS;
+            if (k < d.ntiles) {
+                const uint32_t pos = atomicAdd(&fill[xr[i]][cr[i]], 1u);
+                d.perm[NRES * pos + xr[i]] = k;
+            }
+        }
+    } else {
+        for (int32_t k = tid; k < d.ntiles; k += ROWS_THREADS) {
+            const uint32_t x = d.tile_res[k];
+            const uint32_t pos = atomicAdd(&fill[x][cls[k]], 1u);
+            d.perm[NRES * pos + x] = k;
+        }
     }
     /* the remaining slots of each residue are empty */
     const int32_t per = d.nslots / NRES;
