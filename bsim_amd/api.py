@@ -37,6 +37,7 @@ _SIGS = {
     "bsim_net_add_array": (C.c_int32, [P, C.c_int32, P, P, P]),
     "bsim_net_add_bernoulli": (C.c_int32, [P, C.c_int32, P, C.c_uint64]),
     "bsim_net_add_bernoulli_keyed": (C.c_int32, [P, C.c_int32, P, C.c_uint64, C.c_uint32]),
+    "bsim_net_add_poisson": (C.c_int32, [P, C.c_int32, P, P, P, P, C.c_uint64]),
     "bsim_net_gen_chain": (C.c_int32, [P, P]),
     "bsim_net_gen_chains": (C.c_int32, [P, P, C.c_int32, P]),
     "bsim_net_connect": (C.c_int64, [P, C.c_int64, P, P, P, P, P, P]),
@@ -216,6 +217,20 @@ class Network:
             _err(r, "bsim_net_add_bernoulli")
         return r
 
+    def add_poisson(self, rate, start, duration, seed, taus=None):
+        """Poisson_spikes(rate, refract, startTime, duration) sources with
+        seeded draws (src/neuron/poisson/PoissonNeuron.cpp:13-63); rate,
+        start, duration per neuron; taus [n, 2] composite tau_syn_E/I."""
+        r_ = np.ascontiguousarray(rate, np.float32)
+        n = len(r_)
+        s_ = np.ascontiguousarray(np.broadcast_to(np.asarray(start, np.float32), (n,)))
+        d_ = np.ascontiguousarray(np.broadcast_to(np.asarray(duration, np.float32), (n,)))
+        t_ = None if taus is None else np.ascontiguousarray(taus, np.float32).reshape(n, 2)
+        r = lib().bsim_net_add_poisson(self._h, n, _p(r_), _p(s_), _p(d_), _p(t_), int(seed))
+        if r < 0:
+            _err(r, "bsim_net_add_poisson")
+        return r
+
     def connect(self, pre, post, w, d, stype=None, tau=None):
         pre = np.ascontiguousarray(pre, np.int32)
         post = np.ascontiguousarray(post, np.int32)
@@ -248,6 +263,10 @@ class Network:
                 cyc = np.asarray(spec.arr_t[off[0]:off[-1]], np.int32)
                 taus = np.asarray(spec.lif[i:j, 10:12], np.float32)
                 net.add_array(off - off[0], cyc, taus)
+            elif kind[i] == netspec.POISSON:
+                pp = np.asarray(spec.pois[i:j], np.float64).astype(np.float32)
+                net.add_poisson(pp[:, 0], pp[:, 1], pp[:, 2], spec.seed,
+                                np.asarray(spec.lif[i:j, 10:12], np.float32))
             else:
                 net.add_bernoulli(np.asarray(spec.bern_p[i:j], np.uint32), spec.seed)
             i = j

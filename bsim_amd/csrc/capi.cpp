@@ -4,6 +4,7 @@
  */
 #include <hip/hip_runtime.h>
 
+#include <cmath>
 #include <cstdio>
 #include <cstring>
 #include <new>
@@ -145,6 +146,40 @@ int32_t bsim_net_add_bernoulli_keyed(bsim_network *net, int32_t n, const uint32_
         h.bern_p.insert(h.bern_p.end(), p, p + n);
         h.bern_seed.insert(h.bern_seed.end(), (size_t)n, seed);
         for (int32_t i = 0; i < n; i++) h.bern_key.push_back(first_key + (uint32_t)i);
+        h.seed = seed;
+    } catch (...) {
+        return fail(BSIM_E_NOMEM, "out of host memory");
+    }
+    return first;
+}
+
+int32_t bsim_net_add_poisson(bsim_network *net, int32_t n, const float *rate, const float *start,
+                             const float *duration, const float *taus, uint64_t seed)
+{
+    if (!net || n < 0 || (n > 0 && (!rate || !start || !duration)))
+        return fail(BSIM_E_ARG, "bsim_net_add_poisson: bad argument");
+    auto &h = net->net;
+    if ((int64_t)h.n() + n > 0x7FFFFFFF) return fail(BSIM_E_ARG, "too many neurons");
+    for (int32_t i = 0; i < n; i++) {
+        if (!(rate[i] >= 0.0f && rate[i] <= BSIM_POISSON_MAX_RATE))
+            return fail(BSIM_E_ARG, "Poisson rate (mean cycles between spikes, minus one) must be in [0, 1e6]");
+        if (!(start[i] >= 0.0f && duration[i] >= 0.0f && std::isfinite(start[i] + duration[i])))
+            return fail(BSIM_E_ARG, "Poisson start time and duration must be finite and >= 0");
+    }
+    const int32_t first = h.n();
+    try {
+        h.kind.insert(h.kind.end(), (size_t)n, (int8_t)BSIM_KIND_POISSON);
+        for (int32_t i = 0; i < n; i++) {
+            float row[bsim::kLifParams] = {0};
+            row[10] = taus ? taus[2 * i] : 1e-3f;       // CompositeNeuron's default taus
+            row[11] = taus ? taus[2 * i + 1] : 1e-3f;   // (CompositeNeuron.h:160)
+            h.lif.insert(h.lif.end(), row, row + bsim::kLifParams);
+            h.pois.push_back({first + i, rate[i], start[i], start[i] + duration[i]});
+        }
+        h.arr_off.insert(h.arr_off.end(), (size_t)n, h.arr_off.back());
+        h.bern_p.insert(h.bern_p.end(), (size_t)n, 0u);
+        h.bern_seed.insert(h.bern_seed.end(), (size_t)n, seed);
+        for (int32_t i = 0; i < n; i++) h.bern_key.push_back((uint32_t)(first + i));
         h.seed = seed;
     } catch (...) {
         return fail(BSIM_E_NOMEM, "out of host memory");
@@ -345,6 +380,7 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
     UP(bern_p, L.bern_p);
     UP(bern_seed, L.bern_seed);
     UP(bern_key, L.bern_key);
+    UP(pois_tab, L.pois_tab);
     UP(tile_base, L.tile_base);
     UP(tile_nsyn, L.tile_nsyn);
     UP(tile_res, L.tile_res);

@@ -422,6 +422,10 @@ extern "C" int32_t bsim_net_export_neurons(const bsim_network *netp, int8_t *kin
         return BSIM_E_ARG;
     }
     const bsim::HostNet &h = netp->net;
+    if (!h.pois.empty()) {
+        g_bsim_err = "bsim_net_export_neurons: networks with Poisson sources are not exported";
+        return BSIM_E_ARG;
+    }
     if (kind) std::copy(h.kind.begin(), h.kind.end(), kind);
     if (lif) std::copy(h.lif.begin(), h.lif.end(), lif);
     if (bern_p) std::copy(h.bern_p.begin(), h.bern_p.end(), bern_p);

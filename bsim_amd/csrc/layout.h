@@ -49,7 +49,13 @@
 #define BSIM_TILE_ALIGN (64 * BSIM_NWAVE)   /* parts of whole 64-lane wavefronts */
 #define BSIM_MAX_TILE (8 * BSIM_TILE_ALIGN) /* at most 8 accumulator slots per lane */
 
-enum { BSIM_KIND_LIF = 0, BSIM_KIND_ARRAY = 1, BSIM_KIND_BERNOULLI = 2 };
+enum { BSIM_KIND_LIF = 0, BSIM_KIND_ARRAY = 1, BSIM_KIND_BERNOULLI = 2, BSIM_KIND_POISSON = 3 };
+
+/* Poisson sources: draw X ~ Poisson(rate) from a 32-bit uniform u by the
+ * table tab = pois_tab + bern_p[i] of its rate: tab[0] = J entries, then
+ * tab[1 + j] = floor(P(X <= j) * 2^32) for j < J - 1; X is the smallest j
+ * with u < tab[1 + j], or J - 1 when there is none. */
+#define BSIM_POISSON_MAX_RATE 1.0e6f
 
 typedef struct bsim_range {
     int32_t lo, hi;     /* pre neurons [lo, hi) of one chunk of a source range of the tile */
@@ -89,6 +95,8 @@ typedef struct bsim_dev {
     const uint32_t *bern_p;
     const uint64_t *bern_seed;  /* Bernoulli draw of neuron i at cycle t: */
     const uint32_t *bern_key;   /*   hash(bern_seed[i], bern_key[i], t) < bern_p[i] */
+    const uint32_t *pois_tab;   /* Poisson sources: inverse-CDF tables (bern_p = offset); their
+                                   next firing cycle and end cycle live in arr_cur / arr_end */
     /* neuron state */
     float *v, *ie, *ii;
     int32_t *refrac;

@@ -7,6 +7,8 @@
 #include "network.h"
 
 #include <math.h>
+
+#include <cmath>
 #include <omp.h>
 
 #include <algorithm>
@@ -169,9 +171,43 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
             L.bern_p[i] = net.bern_p[i];
             L.bern_seed[i] = net.bern_seed[i];
             L.bern_key[i] = net.bern_key[i];
+        } else if (net.kind[i] == BSIM_KIND_POISSON) {
+            L.bern_seed[i] = net.bern_seed[i];
+            L.bern_key[i] = net.bern_key[i];
         } else {
             err = "unknown neuron kind";
             return -1;
+        }
+    }
+    /* Poisson sources: PoissonNeuron::reset (src/neuron/poisson/PoissonNeuron.cpp:37-45)
+     * -- first firing cycle int(startTime / dt) + X, end cycle int(endTime / dt);
+     * the first draw uses t = 0xFFFFFFFF, a cycle the simulation never reaches.
+     * One table per distinct rate. */
+    {
+        std::map<uint32_t, uint32_t> tab_of;     /* rate bits -> table offset */
+        for (const auto &p : net.pois) {
+            uint32_t bits;
+            memcpy(&bits, &p.rate, 4);
+            auto it = tab_of.find(bits);
+            if (it == tab_of.end()) {
+                if (L.pois_tab.size() > 0xFFFFFFFFull) {
+                    err = "too many distinct Poisson rates";
+                    return -1;
+                }
+                it = tab_of.emplace(bits, (uint32_t)L.pois_tab.size()).first;
+                poisson_table(p.rate, L.pois_tab);
+            }
+            const int32_t i = p.id;
+            L.bern_p[i] = it->second;
+            const float c0 = p.start / dt, c1 = p.end / dt;
+            if (!(c0 < 2.0e9f) || !(c1 < 2.0e9f)) {
+                err = "Poisson source start or end beyond 2e9 cycles";
+                return -1;
+            }
+            const uint32_t x = poisson_pick(&L.pois_tab[it->second],
+                                            source_hash(net.bern_seed[i], net.bern_key[i], 0xFFFFFFFFu));
+            L.arr_cur0[i] = (int32_t)std::min<int64_t>((int64_t)(int32_t)c0 + x, 0x7FFFFFFF);
+            L.arr_end[i] = (int32_t)c1;
         }
     }
     if (net.arr_t.size() > 0x7FFFFFFF) {
@@ -512,6 +548,45 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
             }
     }
     return 0;
+}
+
+void poisson_table(float rate, std::vector<uint32_t> &tab)
+{
+    /* P(X = j) = exp(-lam + j log lam - lgamma(j + 1)), summed in double; the
+     * table ends where the remaining mass is below 2^-32 for any rate */
+    const double lam = (double)rate;
+    const uint32_t J = (uint32_t)std::ceil(lam + 12.0 * std::sqrt(lam) + 16.0);
+    const size_t at = tab.size();
+    tab.resize(at + 1 + J);
+    tab[at] = J;
+    double cdf = 0.0;
+    for (uint32_t j = 0; j < J; j++) {
+        const double lp = (lam > 0.0) ? -lam + (double)j * std::log(lam) - std::lgamma((double)j + 1.0)
+                                      : (j == 0 ? 0.0 : -INFINITY);
+        cdf += std::exp(lp);
+        const double q = std::floor(cdf * 4294967296.0);
+        tab[at + 1 + j] = q >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)q;
+    }
+}
+
+uint32_t poisson_pick(const uint32_t *tab, uint32_t u)
+{
+    uint32_t lo = 0, hi = tab[0] - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (u < tab[1 + mid]) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
+uint32_t source_hash(uint64_t seed, uint32_t key, uint32_t t)
+{
+    uint64_t z = seed ^ (((uint64_t)key << 32) | (uint64_t)t);
+    z += 0x9E3779B97F4A7C15ull;
+    z = (z ^ (z >> 30)) * 0xBF58476D1CE4E5B9ull;
+    z = (z ^ (z >> 27)) * 0x94D049BB133111EBull;
+    z ^= z >> 31;
+    return (uint32_t)(z >> 32);
 }
 
 }  // namespace bsim

@@ -36,6 +36,11 @@ struct HostNet {
     std::vector<uint64_t> bern_seed;   /* per neuron: seed and key of its Bernoulli */
     std::vector<uint32_t> bern_key;    /* draws, hash(seed, key, t) (0 for others) */
     uint64_t seed = 0;                 /* seed of the last add_bernoulli (export) */
+    struct PoissonSource {             /* PoissonNeuron(rate, refract, startTime, duration) */
+        int32_t id;
+        float rate, start, end;        /* end = startTime + duration, as the reference stores it */
+    };
+    std::vector<PoissonSource> pois;   /* Poisson sources, ascending ids */
     std::vector<int32_t> pre, post;
     std::vector<float> w, d;
     std::vector<int8_t> stype;
@@ -54,6 +59,7 @@ struct Layout {
     std::vector<uint32_t> bern_p;
     std::vector<uint64_t> bern_seed;
     std::vector<uint32_t> bern_key;
+    std::vector<uint32_t> pois_tab;          /* inverse-CDF tables of the Poisson sources' rates */
     std::vector<int64_t> tile_syn;           /* [ntiles + 1] tile-major synapse counts (prefix) */
     std::vector<uint16_t> syn_tgt;
     std::vector<float> syn_w;
@@ -89,6 +95,14 @@ int32_t max_delay_steps(const HostNet &net, float dt);
 
 /* Returns 0 or a negative code with a message in err. */
 int build_layout(const HostNet &net, float dt, int32_t tile, Layout &out, std::string &err);
+
+/* Appends the inverse-CDF table of Poisson(rate) to tab (format in layout.h). */
+void poisson_table(float rate, std::vector<uint32_t> &tab);
+/* X ~ Poisson from the uniform u with a table made by poisson_table. */
+uint32_t poisson_pick(const uint32_t *tab, uint32_t u);
+/* The sources' counter-based hash: splitmix64's finalizer over
+ * seed ^ (key << 32 | t), upper 32 bits. */
+uint32_t source_hash(uint64_t seed, uint32_t key, uint32_t t);
 
 }  // namespace bsim
 

@@ -85,6 +85,18 @@ __device__ __forceinline__ uint32_t bern_hash(uint64_t seed, uint32_t id, uint32
     return (uint32_t)(z >> 32);
 }
 
+/* X ~ Poisson from the uniform u by binary search of the rate's inverse-CDF
+ * table (layout.h) */
+__device__ __forceinline__ uint32_t poisson_pick(const uint32_t *tab, uint32_t u)
+{
+    uint32_t lo = 0, hi = tab[0] - 1;
+    while (lo < hi) {
+        const uint32_t mid = (lo + hi) >> 1;
+        if (u < tab[1 + mid]) hi = mid; else lo = mid + 1;
+    }
+    return lo;
+}
+
 __device__ __forceinline__ uint64_t ballot64(bool p) { return __builtin_amdgcn_ballot_w64(p); }
 
 __device__ __forceinline__ uint32_t readlane(uint32_t x, int32_t l)
@@ -612,6 +624,17 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
                     if (c < d.arr_end[nid] && d.arr_t[c] <= t) {
                         fired = true;
                         d.arr_cur[nid] = c + 1;
+                    }
+                } else if (kind == BSIM_KIND_POISSON) {
+                    /* PoissonNeuron::update (src/neuron/poisson/PoissonNeuron.cpp:52-63):
+                     * fire at the drawn cycle before the end cycle, then draw
+                     * the next one 1 + X cycles later */
+                    const int32_t nx = d.arr_cur[nid];
+                    if (t == nx && nx < d.arr_end[nid]) {
+                        fired = true;
+                        const uint32_t x = poisson_pick(d.pois_tab + d.bern_p[nid],
+                                                        bern_hash(d.bern_seed[nid], d.bern_key[nid], (uint32_t)t));
+                        d.arr_cur[nid] = (int32_t)min((int64_t)nx + 1 + x, (int64_t)0x7FFFFFFF);
                     }
                 } else {
                     fired = bern_hash(d.bern_seed[nid], d.bern_key[nid], (uint32_t)t) < d.bern_p[nid];

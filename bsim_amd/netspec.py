@@ -17,10 +17,12 @@ tau_syn_E, tau_syn_I, v_thresh, i_offset), tau_syn_E, tau_syn_I)``
     10 composite tau_syn_E  11 composite tau_syn_I
 
 Neuron kinds: ``LIF`` (LIF_brian), ``ARRAY`` (Array_spikes: fires at the
-listed cycles, src/neuron/array/ArrayNeuron.cpp:115-128) and ``BERNOULLI``
-(a seeded per-cycle Bernoulli source, this build's deterministic stand-in for
-the reference's clock-seeded Poisson_spikes, src/neuron/poisson/
-PoissonNeuron.cpp:25-26).
+listed cycles, src/neuron/array/ArrayNeuron.cpp:115-128), ``BERNOULLI``
+(a seeded per-cycle Bernoulli source) and ``POISSON`` (Poisson_spikes with the
+reference's CPU semantics, src/neuron/poisson/PoissonNeuron.cpp:37-63: first
+spike at int(start/dt) + X, then every 1 + X cycles before int(end/dt), X ~
+Poisson(rate) drawn from seeded hashes instead of the reference's
+clock-seeded engine, PoissonNeuron.cpp:25-26).
 """
 from __future__ import annotations
 
@@ -28,7 +30,7 @@ from dataclasses import dataclass, field
 
 import numpy as np
 
-LIF, ARRAY, BERNOULLI = 0, 1, 2
+LIF, ARRAY, BERNOULLI, POISSON = 0, 1, 2, 3
 LIF_NPARAM = 12
 
 
@@ -46,6 +48,7 @@ class NetSpec:
     arr_t: np.ndarray | None = None    # int32 spike cycles
     bern_p: np.ndarray | None = None   # uint32 [n] threshold of the source hash
     seed: int = 0
+    pois: np.ndarray | None = None     # float [n, 3] rate, startTime, duration (Poisson rows)
     pops: list = field(default_factory=list)  # [(start, count)] per population
 
     @property
@@ -75,13 +78,18 @@ class Builder:
         self.lifs = []
         self.arr = []           # per neuron list of cycles (ARRAY) or None
         self.bern = []          # per neuron threshold
+        self.pois = []          # per population [cnt, 3] Poisson parameters or None
         self.pre, self.post, self.w, self.d, self.stype, self.tau = [], [], [], [], [], []
         self.pops = []
         self.n = 0
 
-    def population(self, rows, kind=LIF, spikes=None, bern_p=None):
+    def population(self, rows, kind=LIF, spikes=None, bern_p=None, pois=None):
+        """pois: (rate, startTime, duration), each a scalar or one per neuron,
+        for kind POISSON."""
         rows = np.atleast_2d(np.asarray(rows, np.float64))
         cnt = rows.shape[0]
+        self.pois.append(None if pois is None else np.stack(
+            [np.broadcast_to(np.asarray(x, np.float64), (cnt,)) for x in pois], axis=1))
         start = self.n
         self.kinds.append(np.full(cnt, kind, np.int8))
         self.lifs.append(rows)
@@ -122,12 +130,16 @@ class Builder:
         else:
             arr_off, arr_t = None, None
         cat = lambda xs, dt: (np.concatenate(xs).astype(dt) if xs else np.zeros(0, dt))  # noqa
+        pois = None
+        if any(p is not None for p in self.pois):
+            pois = np.concatenate([np.zeros((len(k), 3)) if p is None else p
+                                   for p, k in zip(self.pois, self.kinds)])
         return NetSpec(kind=kind, lif=lif, pre=cat(self.pre, np.int32),
                        post=cat(self.post, np.int32), w=cat(self.w, wdtype),
                        d=cat(self.d, wdtype), stype=cat(self.stype, np.int8),
                        tau=cat(self.tau, wdtype), arr_off=arr_off, arr_t=arr_t,
                        bern_p=np.concatenate(self.bern) if self.bern else None,
-                       pops=list(self.pops))
+                       pois=pois, pops=list(self.pops))
 
 
 # ---- the reference's own unit networks (test/gpu/lif_unit{1..4}_test.cpp) ----

@@ -25,6 +25,8 @@
  *   CompositeNeuron / LIF_brian / Array_spikes     src/base/CompositeNeuron.h:20-21,
  *                                                  include/Neuron.h:25-31
  *   ArrayNeuron(int *array, int num)               src/neuron/array/ArrayNeuron.h:19-21
+ *   PoissonNeuron(rate, refract, startTime,        src/neuron/poisson/PoissonNeuron.cpp:13-63
+ *                 duration) / Poisson_spikes       (seeded: setSeed; see the class)
  *   SGSim(Network*, dt), run(T), run(T, FireInfo&) src/sim/SingleGPUSimulator.h:12-16,
  *                                                  src/base/SimulatorBase.cpp:29
  *   FireInfo, ArrayInfo                            src/base/SimInfo.h:26-31
@@ -147,6 +149,34 @@ protected:
     std::vector<int> fireTime;
 };
 
+/* PoissonNeuron(rate, refract, startTime, duration)
+ * (src/neuron/poisson/PoissonNeuron.cpp:13-63): first spike at
+ * int(startTime/dt) + X, then 1 + X cycles after each spike, before
+ * int((startTime + duration)/dt); X ~ Poisson(rate).  The reference seeds its
+ * engine from the clock; here the draws are a function of the seed (setSeed,
+ * default 0), the neuron's id and the cycle, so runs repeat exactly.  refract
+ * has no effect on the reference's CPU path and none here. */
+class PoissonNeuron : public NeuronBase {
+public:
+    PoissonNeuron(real rate, real refract, real startTime, real duration)
+        : rate_(rate), refract_(refract), start_(startTime), duration_(duration) {}
+    PoissonNeuron &setSeed(uint64_t seed)
+    {
+        seed_ = seed;
+        return *this;
+    }
+    real rate() const { return rate_; }
+    real refract() const { return refract_; }
+    real startTime() const { return start_; }
+    real duration() const { return duration_; }
+    uint64_t seed() const { return seed_; }
+    static const int kind = 3;
+
+protected:
+    real rate_, refract_, start_, duration_;
+    uint64_t seed_ = 0;
+};
+
 template <class Neuron, class Synapse>
 class CompositeNeuron : public Neuron {
 public:
@@ -162,6 +192,7 @@ private:
 typedef CompositeNeuron<LIFENeuron, StaticSynapse> LIF_brian;
 typedef CompositeNeuron<LIFENeuron, StaticSynapse> IF_curr_exp;
 typedef CompositeNeuron<ArrayNeuron, StaticSynapse> Array_spikes;
+typedef CompositeNeuron<PoissonNeuron, StaticSynapse> Poisson_spikes;
 
 class PopulationBase {
 public:
@@ -204,6 +235,31 @@ inline int32_t add_neurons(bsim_network *net, const std::vector<Array_spikes> &v
                                           cyc.empty() ? NULL : cyc.data(), taus.data());
     check(id, "createPopulation");
     return id;
+}
+
+inline int32_t add_neurons(bsim_network *net, const std::vector<Poisson_spikes> &v, size_t from)
+{
+    int32_t first = -1;
+    for (size_t a = from; a < v.size();) {          /* one call per run of equal seeds */
+        size_t b = a;
+        while (b < v.size() && v[b].seed() == v[a].seed()) b++;
+        const size_t n = b - a;
+        std::vector<float> rate(n), start(n), dur(n), taus(n * 2);
+        for (size_t i = 0; i < n; i++) {
+            const Poisson_spikes &x = v[a + i];
+            rate[i] = x.rate();
+            start[i] = x.startTime();
+            dur[i] = x.duration();
+            taus[i * 2] = x.tauSynE();
+            taus[i * 2 + 1] = x.tauSynI();
+        }
+        const int32_t id = bsim_net_add_poisson(net, (int32_t)n, rate.data(), start.data(), dur.data(),
+                                                taus.data(), v[a].seed());
+        check(id, "createPopulation");
+        if (first < 0) first = id;
+        a = b;
+    }
+    return first;
 }
 }  // namespace bsim_cpp
 

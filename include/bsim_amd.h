@@ -14,6 +14,9 @@
  *                                src/base/Population.h:158-168)
  *      bsim_net_add_array     <- createPopulation<Array_spikes>
  *                                (src/neuron/array/ArrayNeuron.cpp:70-77)
+ *      bsim_net_add_poisson   <- createPopulation<Poisson_spikes>
+ *                                (src/neuron/poisson/PoissonNeuron.cpp:13-63), with
+ *                                seeded draws instead of the clock-seeded engine
  *      bsim_net_add_bernoulli <- (no counterpart: the reference's Poisson_spikes
  *                                is seeded from the clock, PoissonNeuron.cpp:25-26)
  *      bsim_net_connect       <- Network::connect(NeuronBase*, NeuronBase*, w, d,
@@ -93,6 +96,20 @@ int32_t bsim_net_add_bernoulli(bsim_network *net, int32_t n, const uint32_t *p, 
  * counterpart in the reference, whose Poisson sources are clock-seeded.) */
 int32_t bsim_net_add_bernoulli_keyed(bsim_network *net, int32_t n, const uint32_t *p, uint64_t seed,
                                      uint32_t first_key);
+
+/* Appends n Poisson sources with the reference's CPU semantics
+ * (PoissonNeuron.cpp:37-63): source i first fires at cycle
+ * int(start[i] / dt) + X, then 1 + X cycles after each spike, while the
+ * firing cycle is below int((start[i] + duration[i]) / dt); every X is an
+ * independent Poisson(rate[i]) draw (rate = the mean number of cycles
+ * between spikes minus one, as the reference's poisson_distribution takes
+ * it; 0..1e6).  Draws are seeded: X = the inverse CDF of Poisson(rate[i]) at
+ * hash(seed, id, t) (t = the spike's cycle; 0xFFFFFFFF for the first draw),
+ * id = the neuron's global id.  taus (may be NULL) holds the composite
+ * tau_syn_E/I per neuron.  The reference's refract argument has no effect on
+ * its CPU path and is not taken. */
+int32_t bsim_net_add_poisson(bsim_network *net, int32_t n, const float *rate, const float *start,
+                             const float *duration, const float *taus, uint64_t seed);
 
 /* Appends count synapses in the given order (= creation order).  d is the
  * delay in seconds; stype (NULL: all excitatory) and tau (NULL: all 0) as in

@@ -38,7 +38,7 @@
 extern "C" {
 #endif
 
-enum { BSO_LIF = 0, BSO_ARRAY = 1, BSO_BERNOULLI = 2 };
+enum { BSO_LIF = 0, BSO_ARRAY = 1, BSO_BERNOULLI = 2, BSO_POISSON = 3 };
 
 /* Number of REAL values per LIF neuron in the parameter block:
  * v_init v_rest v_reset cm tau_m tau_refrac tau_syn_E tau_syn_I v_thresh
@@ -49,7 +49,7 @@ enum { BSO_LIF = 0, BSO_ARRAY = 1, BSO_BERNOULLI = 2 };
 #define BSO_DECLARE_NET(REAL, SUF)                                            \
 typedef struct bso_net_##SUF {                                                \
     int32_t n;               /* neurons, in creation order                */ \
-    const int8_t *kind;      /* BSO_LIF / BSO_ARRAY / BSO_BERNOULLI         */ \
+    const int8_t *kind;      /* BSO_LIF / ARRAY / BERNOULLI / POISSON       */ \
     const REAL *lif;         /* n * BSO_LIF_NPARAM (rows of non-LIF unused) */ \
     const int64_t *arr_off;  /* n + 1 offsets into arr_t (may be NULL)      */ \
     const int32_t *arr_t;    /* ascending fire cycles per array neuron      */ \
@@ -62,6 +62,9 @@ typedef struct bso_net_##SUF {                                                \
     const REAL *d;           /* delay in seconds                            */ \
     const int8_t *stype;     /* 0 excitatory, 1 inhibitory (may be NULL)    */ \
     const REAL *tau;         /* explicit tau (0 -> composite); may be NULL  */ \
+    const REAL *pois;        /* n * 3: rate, startTime, duration of each    */ \
+                             /* Poisson source (other rows unused; may be   */ \
+                             /* NULL without Poisson sources)               */ \
 } bso_net_##SUF;                                                              \
                                                                               \
 typedef struct bso_out_##SUF {                                                \
@@ -113,6 +116,13 @@ int bso_csr_order_ok(int32_t n, int64_t s, const int32_t *pre,
 /* Deterministic Bernoulli source hash (splitmix64 finalizer over seed, id and
  * cycle); a source fires at cycle t iff bso_hash(seed, id, t) < p. */
 uint32_t bso_hash(uint64_t seed, uint32_t id, uint32_t t);
+
+/* Poisson sources (src/neuron/poisson/PoissonNeuron.cpp:37-63, drawn from
+ * seeded uniforms instead of a clock-seeded engine): X ~ Poisson(rate) is the
+ * smallest j with u < floor(P(X <= j) * 2^32), u = bso_hash(seed, id, t).
+ * bso_poisson_table returns a malloc'd table {J, thresholds[0 .. J-1]}. */
+uint32_t *bso_poisson_table(double rate);
+uint32_t bso_poisson_pick(const uint32_t *tab, uint32_t u);
 
 #ifdef __cplusplus
 }

@@ -10,10 +10,11 @@ oracle/Makefile), a plain-C restatement of BSim's single-threaded CPU simulator:
 * synapse reset   -- src/synapse/static/StaticSynapse.cpp:81-92
 * step order      -- src/net/Network.cpp:308-330
 * array source    -- src/neuron/array/ArrayNeuron.cpp:115-128
+* Poisson source  -- src/neuron/poisson/PoissonNeuron.cpp:37-63 (seeded draws)
 
 A network is passed as an object with numpy attributes (``kind``, ``lif``,
 ``arr_off``, ``arr_t``, ``bern_p``, ``seed``, ``pre``, ``post``, ``w``, ``d``,
-``stype``, ``tau``), the layout of ``bsim_amd.netspec.NetSpec``.
+``stype``, ``tau``, optional ``pois``), the layout of ``bsim_amd.netspec.NetSpec``.
 """
 from __future__ import annotations
 
@@ -77,7 +78,8 @@ def _structs(suf):
     class Net(C.Structure):
         _fields_ = [("n", C.c_int32), ("kind", P), ("lif", P), ("arr_off", P),
                     ("arr_t", P), ("bern_p", P), ("seed", C.c_uint64), ("s", C.c_int64),
-                    ("pre", P), ("post", P), ("w", P), ("d", P), ("stype", P), ("tau", P)]
+                    ("pre", P), ("post", P), ("w", P), ("d", P), ("stype", P), ("tau", P),
+                    ("pois", P)]
 
     class Out(C.Structure):
         _fields_ = [("raster_count", P), ("raster_ids", P), ("raster_cap", C.c_int64),
@@ -118,12 +120,15 @@ class _NetArgs:
                       else np.ascontiguousarray(net.stype, dtype=np.int8))
         self.tau = (None if net.tau is None
                     else np.ascontiguousarray(np.asarray(net.tau).astype(dtype)))
+        pois = getattr(net, "pois", None)
+        self.pois = (None if pois is None
+                     else np.ascontiguousarray(np.asarray(pois, np.float64).astype(dtype)))
         suf = "f32" if self.dtype == np.float32 else "f64"
         Net, _ = _structs(suf)
         self.c = Net(n, _ptr(self.kind), _ptr(self.lif), _ptr(self.arr_off), _ptr(self.arr_t),
                      _ptr(self.bern_p), int(net.seed), len(self.pre), _ptr(self.pre),
                      _ptr(self.post), _ptr(self.w), _ptr(self.d), _ptr(self.stype),
-                     _ptr(self.tau))
+                     _ptr(self.tau), _ptr(self.pois))
 
 
 @dataclass

@@ -148,6 +148,32 @@ uint32_t bso_hash(uint64_t seed, uint32_t id, uint32_t t)
     return (uint32_t)(z >> 32);
 }
 
+/* The Poisson pmf from its definition, P(X = j) = lam^j e^-lam / j!, in log
+ * form; the table stops where the remaining mass is below 2^-32. */
+uint32_t *bso_poisson_table(double lam)
+{
+    uint32_t J = (uint32_t)ceil(lam + 12.0 * sqrt(lam) + 16.0);
+    uint32_t *tab = (uint32_t *)malloc(((size_t)J + 1) * sizeof(uint32_t));
+    if (!tab) return NULL;
+    tab[0] = J;
+    double cdf = 0.0;
+    for (uint32_t j = 0; j < J; j++) {
+        double lp = (lam > 0.0) ? -lam + (double)j * log(lam) - lgamma((double)j + 1.0)
+                                : (j == 0 ? 0.0 : -INFINITY);
+        cdf += exp(lp);
+        double q = floor(cdf * 4294967296.0);
+        tab[1 + j] = q >= 4294967295.0 ? 0xFFFFFFFFu : (uint32_t)q;
+    }
+    return tab;
+}
+
+uint32_t bso_poisson_pick(const uint32_t *tab, uint32_t u)
+{
+    uint32_t j = 0;                   /* linear search: the restatement, not the fast form */
+    while (j + 1 < tab[0] && u >= tab[1 + j]) j++;
+    return j;
+}
+
 int bso_csr_order_ok(int32_t n, int64_t s, const int32_t *pre, const int32_t *post,
                      const int32_t *delay_steps)
 {

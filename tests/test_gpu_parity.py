@@ -248,7 +248,7 @@ def dense_net(n_src=700, n_tgt=300, p_fire=0.9, dmax=3, seed=4):
     return spec
 
 
-@pytest.mark.parametrize("tile", [192, 384, 1536])
+@pytest.mark.parametrize("tile", [192, 384, 768, 1536])
 def test_dense_overflow_paths(tile):
     spec = dense_net()
     steps = 60
