@@ -29,6 +29,32 @@ def test_header_compiles_standalone(tmp_path):
     assert r.returncode == 0, r.stderr
 
 
+def test_poisson_population_compiles(tmp_path):
+    """Poisson_spikes populations (PoissonNeuron(rate, refract, startTime,
+    duration), src/neuron/poisson/PoissonNeuron.cpp:13) build through the
+    header, seeded or not."""
+    src = tmp_path / "pois.cpp"
+    src.write_text("""#include "BSim.h"
+int main()
+{
+    Network net;
+    Population<Poisson_spikes> *drive = net.createPopulation(
+        100, Poisson_spikes(PoissonNeuron(40.0f, 0.0f, 0.0f, 0.5f), 1e-3f, 1e-3f));
+    Poisson_spikes seeded(PoissonNeuron(5.0f, 0.0f, 0.01f, 0.1f), 1e-3f, 1e-3f);
+    seeded.setSeed(7);
+    Population<Poisson_spikes> *more = net.createPopulation(10, seeded);
+    more->addNeuron(seeded);
+    Population<LIF_brian> *p = net.createPopulation(
+        50, LIF_brian(LIFENeuron(0, 0, 0, 1e-1f, 50e-3f, 0, 1e-3f, 1e-3f, 15e-3f, 0), 1e-3f, 1e-3f));
+    net.connect(drive, p, 0.1f, 1e-4f, Excitatory);
+    return drive->getNum() == 100 ? 0 : 1;
+}
+""")
+    r = subprocess.run(["g++", "-std=c++11", "-Wall", "-Wextra", "-Werror", "-fsyntax-only",
+                        "-I", os.path.join(ROOT, "include"), str(src)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def write_input(path, spec, pops, projections, dt, T, monitor):
     """pops: [(first, n)], projections: [(src_pop, dst_pop, w, d)] all-to-all."""
     with open(path, "wb") as f:
