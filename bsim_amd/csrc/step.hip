@@ -206,7 +206,7 @@ struct Step {
  * needed.  Two ways to find each position's synapse index:
  *   R == 1  every sub-row that intersects the step writes the index of each
  *           of its positions into the wavefront's address list (one pass per
- *           two positions of the window's longest sub-row);
+ *           four positions of the window's longest sub-row);
  *   R >= 2  (marks) every such sub-row writes only its first position in the
  *           step: its synapse base into the address list and a bit into the
  *           step's position mask; each position then takes the base of the
@@ -234,9 +234,11 @@ __device__ __forceinline__ void load_step(const Shared &S, const uint32_t *rows,
                     const uint32_t n = hi > lo ? hi - lo : 0u;
                     const uint32_t a0 = W.base[r] + lo;
                     uint32_t *dst = al + (lo - b);
-                    for (uint32_t x = 0; ballot64(x < n) != 0ull; x += 2) {
+                    for (uint32_t x = 0; ballot64(x < n) != 0ull; x += 4) {
                         if (x < n) dst[x] = a0 + x;
                         if (x + 1 < n) dst[x + 1] = a0 + x + 1;
+                        if (x + 2 < n) dst[x + 2] = a0 + x + 2;
+                        if (x + 3 < n) dst[x + 3] = a0 + x + 3;
                     }
                 } else if (hi > lo) {
                     const uint32_t x = lo - b;
