@@ -80,6 +80,27 @@ int main(int argc, char **argv)
     rd(f, &nmon, 1);
     std::vector<int> mon((size_t)nmon);
     rd(f, mon.data(), (size_t)nmon);
+    /* optional: Poisson_spikes drive populations, created after the LIF
+     * populations, each projecting all-to-all onto one of them */
+    int32_t npp = 0;
+    if (fread(&npp, sizeof(npp), 1, f) != 1) npp = 0;
+    for (int k = 0; k < npp; k++) {
+        int32_t n, tgt;
+        float par[3];
+        uint64_t seed;
+        rd(f, &n, 1);
+        rd(f, par, 3);
+        rd(f, &seed, 1);
+        rd(f, &tgt, 1);
+        Poisson_spikes templ(PoissonNeuron(par[0], 0.0f, par[1], par[2]), 1e-3f, 1e-3f);
+        templ.setSeed(seed);
+        Population<Poisson_spikes> *pp = c.createPopulation(n, templ);
+        const size_t m = (size_t)n * (size_t)pops[tgt]->getNum();
+        std::vector<real> w(m), dl(m);
+        rd(f, w.data(), m);
+        rd(f, dl.data(), m);
+        c.connect(pp, pops[tgt], w.data(), dl.data(), NULL, (int)m);
+    }
     fclose(f);
 
     if (chdir(argv[2]) != 0) {

@@ -120,3 +120,59 @@ def test_dropin_program_matches_reference_c0(tmp_path):
     assert r.returncode == 0, r.stdout + r.stderr
     fc2 = np.array((plain / "GFire.count").read_text().split(), np.int32)
     np.testing.assert_array_equal(fc2, gold["fire_count"])
+
+
+@pytest.mark.gpu
+def test_dropin_poisson_drive_matches_oracle(tmp_path):
+    """A BSim program with a Poisson_spikes drive population (seeded through
+    PoissonNeuron::setSeed) driving two LIF populations, run through
+    include/BSim.h: its raster and GFire.count equal the oracle's."""
+    if not os.path.exists(BIN):
+        pytest.fail("tests/cpp/dropin_run is not built (run __graft_entry__.build())")
+    rng = np.random.default_rng(21)
+    n0, n1, m = 300, 200, 120
+    rate, start, dur, seed = 20.0, 0.002, 0.03, 9
+    lif = netspec.lif_row(0.0, 0.0, 0.0, 0.02, 0.02, 2e-4, 5e-3, 8e-3, 15.0, 0.0)
+    b = netspec.Builder()
+    for cnt in (n0, n1):
+        rows = np.tile(lif, (cnt, 1))
+        rows[:, 9] = rng.uniform(10.0, 14.0, cnt).astype(np.float32)
+        b.population(rows)
+    prow = np.zeros(netspec.LIF_NPARAM)
+    prow[10] = prow[11] = 1e-3                     # the composite's default taus
+    b.population(np.tile(prow, (m, 1)), kind=netspec.POISSON, pois=(rate, start, dur))
+    w01 = rng.uniform(-0.3, 0.6, n0 * n1).astype(np.float32)
+    d01 = (rng.integers(1, 4, n0 * n1) * np.float32(netspec.DT)).astype(np.float32)
+    wp = rng.uniform(0.0, 2.0, m * n0).astype(np.float32)
+    dp = (rng.integers(1, 3, m * n0) * np.float32(netspec.DT)).astype(np.float32)
+    k = np.arange(n0 * n1)
+    b.connect(k // n1, n0 + k % n1, w01, d01, np.zeros(n0 * n1, np.int8))   # type NULL
+    k = np.arange(m * n0)
+    b.connect(n0 + n1 + k // n0, k % n0, wp, dp, np.zeros(m * n0, np.int8))
+    spec = b.build()
+    spec.seed = seed
+    steps = 400
+    dt = np.float32(netspec.DT)
+    T = np.float32(steps * float(dt))
+    assert int(np.round(np.float32(T / dt))) == steps
+    inp = tmp_path / "pois.bin"
+    write_input(inp, spec, [(0, n0), (n0, n1)], [(0, 1, w01, d01)], dt, T, [])
+    with open(inp, "ab") as f:
+        f.write(struct.pack("<i", 1))
+        f.write(struct.pack("<i", m))
+        f.write(np.array([rate, start, dur], np.float32).tobytes())
+        f.write(struct.pack("<Q", seed))
+        f.write(struct.pack("<i", 0))
+        f.write(wp.tobytes())
+        f.write(dp.tobytes())
+    r = subprocess.run([BIN, str(inp), str(tmp_path)], capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    o = oracle.run(spec, netspec.DT, steps)
+    lines = (tmp_path / "GSim.log").read_text().split("\n")[:steps]
+    counts = np.array([len(l.split()) for l in lines])
+    ids = np.array([int(x) for l in lines for x in l.split()], np.int32)
+    np.testing.assert_array_equal(counts, o.raster_count)
+    np.testing.assert_array_equal(ids, o.raster_ids)
+    fc = np.array((tmp_path / "GFire.count").read_text().split(), np.int32)
+    np.testing.assert_array_equal(fc, o.fire_count)
+    assert fc[n0 + n1:].sum() > 0 and fc[:n0].sum() > 0
