@@ -302,7 +302,9 @@ __device__ __forceinline__ void add_event(Acc<S_> &acc, uint2 ev, bool valid)
     const bool exc = x >= 0.0f;
 #pragma unroll
     for (int j = 0; j < S_; j++) {
-        const bool mine = valid && (S_ == 1 || slot == (uint32_t)j);   /* S_ == 1: slot is 0 */
+        /* with one slot per lane an owned event's local target is below 64:
+         * its slot is 0 and needs no test */
+        const bool mine = valid && (S_ == 1 || slot == (uint32_t)j);
         const float ae = acc.e[j] + x, ai = acc.i[j] + x;
         acc.e[j] = (mine && exc) ? ae : acc.e[j];
         acc.i[j] = (mine && !exc) ? ai : acc.i[j];
