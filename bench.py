@@ -13,8 +13,11 @@ SGSim loop body: LIF update, spike compaction, delayed delivery, synaptic
 accumulation).  The default workload is the standard feed-forward network
 at ~100 Hz (64 x 10,000 LIF, ~6.3e8 synapses, delays 1..6 steps, dt 0.1 ms),
 built from the seeded generator of bsim_amd.netspec and uploaded before the
-timed region.  W cycles run untimed, then exactly K cycles are timed between
-two stream synchronisations.  One JSON line is printed:
+timed region.  The network is first pre-rolled a fixed number of cycles
+(--preroll, default 1,000 = 0.1 s simulated, independent of --warmup) so that
+the timed cycles are in the configured firing regime rather than the cold
+start; then W cycles run untimed and exactly K cycles are timed between two
+stream synchronisations.  One JSON line is printed:
 
   value        synaptic events (deliveries) per second over the K cycles
   roofline     the step kernel's algorithmic bytes (8 B per delivered event:
@@ -25,8 +28,10 @@ two stream synchronisations.  One JSON line is printed:
                HBM3E).  The cycle's second kernel (active-row lists) is timed
                the same way and reported beside it.
   cpu_baseline the oracle's event-driven CSR form (oracle/stsim.c) on the
-               host's cores, timed over a window of the same network after
-               the same warm-up
+               host's cores, timed over the same cycles of the same network
+               that the GPU timed (the first --cpu-steps of them when K is
+               larger); the deliveries it counts over those cycles are
+               compared with the GPU's ("events_match")
 
 Rasters are not recorded in the timed region (recording costs one 8-byte
 store per 64 neurons per cycle); bit-exactness is established by the gpu
@@ -37,7 +42,8 @@ rocprofv3 counter runs (FETCH_SIZE, then WRITE_SIZE) of this script, each
 scaled by what the same counter reads for --copy-probe, a plain copy kernel of
 known size; --pmc-summary prints the per-dispatch averages of such a run.  The
 figures are read from profiles/step_kernel_<workload>_traffic.txt when that
-file exists for the same workload and tile (null otherwise).
+file exists for the same workload and tile and was measured at a load within
+10 % of this run's deliveries per launch (null otherwise).
 """
 from __future__ import annotations
 
@@ -125,11 +131,13 @@ def cpu_reference(workload, steps):
                       f"is visited every cycle)"}
 
 
-def cpu_baseline(spec, warmup, steps, threads):
+def cpu_baseline(spec, start, steps, threads):
+    """The oracle's CSR form over cycles [0, start + steps), timed over
+    [start, start + steps): (events/s, events in the window, seconds)."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import oracle  # noqa: E402  (the oracle is only the checker / baseline)
-    r = oracle.run(spec, netspec.DT, warmup + steps, mode="csr", tile=2048, threads=threads,
-                   raster_cap=1 << 26, time_from=warmup)
+    r = oracle.run(spec, netspec.DT, start + steps, mode="csr", tile=2048, threads=threads,
+                   raster_cap=1 << 26, time_from=start)
     return r.events_from / r.wall_s, r.events_from, r.wall_s
 
 
@@ -176,22 +184,29 @@ def pmc_summary(path, kernel_re, last=0):
     print(json.dumps({"files": len(files), "kernel": kernel_re, "last": last, "counters": out}))
 
 
-def read_traffic(workload, tile):
-    """HBM bytes per launch of the step kernel from the committed counter summary."""
+def read_traffic(workload, tile, events_per_launch):
+    """(HBM bytes per launch of the step kernel, deliveries per launch of the
+    profiled cycles) from the committed counter summary, or (None, profiled
+    deliveries) when the profile's tile differs or its load is not within 10 %
+    of this run's events_per_launch (traffic depends on the firing regime)."""
     fn = os.path.join(ROOT, "profiles", f"step_kernel_{workload}_traffic.txt")
     if not os.path.exists(fn):
-        return None
+        return None, None
     kv = {}
     for line in open(fn):
         if ":" in line and not line.startswith("#"):
             k, v = line.split(":", 1)
             kv[k.strip()] = v.strip()
     try:
+        prof_ev = float(kv["events_per_launch"]) if "events_per_launch" in kv else \
+            float(kv["algorithmic_bytes_per_launch"]) / BYTES_PER_EVENT
         if int(kv["tile"]) != tile:
-            return None
-        return float(kv["hbm_bytes_per_launch"])
+            return None, prof_ev
+        if not (abs(prof_ev - events_per_launch) <= 0.1 * max(events_per_launch, 1.0)):
+            return None, prof_ev
+        return float(kv["hbm_bytes_per_launch"]), prof_ev
     except (KeyError, ValueError):
-        return None
+        return None, None
 
 
 def bench_sweep(a):
@@ -414,10 +429,13 @@ def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10000)
-    ap.add_argument("--warmup", type=int, default=1000)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--preroll", type=int, default=1000,
+                    help="cycles run before the warm-up to reach the firing regime (setup)")
     ap.add_argument("--workload", default="c1")
     ap.add_argument("--tile", type=int, default=0)
-    ap.add_argument("--cpu-steps", type=int, default=1000)
+    ap.add_argument("--cpu-steps", type=int, default=1000,
+                    help="largest number of timed cycles the CPU baseline repeats")
     ap.add_argument("--cpu-threads", type=int, default=0)
     ap.add_argument("--ref-steps", type=int, default=100,
                     help="cycles of the reference CPU simulator's timed sample")
@@ -479,18 +497,39 @@ def main():
     info = sim.info()
     print(f"[bench] built+uploaded in {t_build:.1f}s: {info}", file=sys.stderr, flush=True)
 
+    t0 = time.time()
+    sim.step(a.preroll)
+    sim.sync()
+    ev_pre = sim.events()
+    print(f"[bench] pre-rolled {a.preroll} cycles in {time.time() - t0:.2f}s ({ev_pre} events)",
+          file=sys.stderr, flush=True)
     sim.step(a.warmup)
     sim.sync()
+    start = a.preroll + a.warmup
+    # the first min(K, --cpu-steps) timed cycles are the CPU baseline's window;
+    # their deliveries are read at a stream synchronisation inside the timed
+    # region (one extra host wait when K is larger)
+    k_cpu = 0 if a.no_cpu else min(a.steps, a.cpu_steps)
     ev0 = sim.events()
     t0 = time.perf_counter()
-    sim.step(a.steps)
+    ev_cpu_window = None
+    if 0 < k_cpu < a.steps:
+        sim.step(k_cpu)
+        sim.sync()
+        ev_cpu_window = sim.events() - ev0
+        sim.step(a.steps - k_cpu)
+    else:
+        sim.step(a.steps)
     sim.sync()
     t1 = time.perf_counter()
     ev1 = sim.events()
     wall = t1 - t0
     events = ev1 - ev0
+    if ev_cpu_window is None:
+        ev_cpu_window = events
     value = events / wall
-    print(f"[bench] timed {a.steps} cycles in {wall:.4f}s, {events} events", file=sys.stderr,
+    print(f"[bench] timed {a.steps} cycles (cycles {start}..{start + a.steps}) in {wall:.4f}s, "
+          f"{events} events ({events / max(a.steps, 1):.4g} per cycle)", file=sys.stderr,
           flush=True)
 
     # live kernel timing: HIP events around each kernel of a further pass,
@@ -505,18 +544,21 @@ def main():
     print(f"[bench] step kernel mean {step_us:.2f} us, row-list kernel mean {rows_us:.2f} us "
           f"over {R} launches, {ev_per_launch:.0f} events/launch", file=sys.stderr, flush=True)
     sim.close()
+    traffic, traffic_ev = read_traffic(a.workload, info["tile"], ev_per_launch)
 
     cpu = None
     cpu_ref = None
     if not a.no_cpu:
         threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
-        cw = min(a.warmup, 1000)
-        v, ev, secs = cpu_baseline(spec, cw, a.cpu_steps, threads)
+        v, ev, secs = cpu_baseline(spec, start, k_cpu, threads)
         cpu = {"value": v, "unit": "synaptic events/s", "cores": threads, "kind": "port",
-               "sample": f"oracle CSR form, cycles {cw}..{cw + a.cpu_steps} of the same "
-                         f"network ({ev} events in {secs:.2f}s)", "host": host_info()}
-        print(f"[bench] cpu baseline {v:.3g} events/s on {threads} threads", file=sys.stderr,
-              flush=True)
+               "sample": f"oracle CSR form (OpenMP), cycles {start}..{start + k_cpu} of the same "
+                         f"network = the GPU's first {k_cpu} timed cycles ({ev} events in "
+                         f"{secs:.3f}s; cycles 0..{start} run untimed first)",
+               "events_match": bool(ev == ev_cpu_window), "gpu_over_cpu": value / v,
+               "host": host_info()}
+        print(f"[bench] cpu baseline {v:.3g} events/s on {threads} threads over the same "
+              f"{k_cpu} cycles; events {ev} (GPU {ev_cpu_window})", file=sys.stderr, flush=True)
         cpu_ref = cpu_reference(a.workload, a.ref_steps)
         if cpu_ref:
             print(f"[bench] reference CPU path {cpu_ref['value']:.3g} events/s ({cpu_ref['sample']})",
@@ -537,10 +579,13 @@ def main():
         "data": "synthetic (seeded generator)",
         "config": {"workload": desc, "name": a.workload, "neurons": n_neurons,
                    "synapses": n_syn, "dt_s": netspec.DT, "tile": info["tile"],
+                   "preroll": a.preroll, "timed_cycles": [start, start + a.steps],
+                   "events_per_cycle": events / max(a.steps, 1),
                    "simulated_s": a.steps * netspec.DT, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,
                      "unit": "GB/s", "frac": achieved / HBM_PEAK_GBS,
-                     "traffic": read_traffic(a.workload, info["tile"]),
+                     "traffic": traffic,
+                     "traffic_profile_events_per_launch": traffic_ev,
                      "kernel": "bsim_step_kernel", "mean_launch_us": step_us,
                      "events_per_launch": ev_per_launch,
                      "row_list_kernel_us": rows_us},

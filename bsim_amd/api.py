@@ -16,6 +16,7 @@ implementation.
 from __future__ import annotations
 
 import ctypes as C
+import math
 import os
 
 import numpy as np
@@ -50,6 +51,8 @@ _SIGS = {
     "bsim_sim_cycle": (C.c_int32, [P]),
     "bsim_sim_read_raster": (C.c_int64, [P, P, P, C.c_int64, P]),
     "bsim_sim_read_trace": (C.c_int32, [P, P]),
+    "bsim_sim_enable_state_trace": (C.c_int32, [P]),
+    "bsim_sim_read_state_trace": (C.c_int32, [P, P, P, P, P]),
     "bsim_sim_read_state": (C.c_int32, [P, P, P, P, P, P]),
     "bsim_sim_enable_input_log": (C.c_int32, [P]),
     "bsim_sim_read_input_log": (C.c_int32, [P, P, P]),
@@ -59,6 +62,7 @@ _SIGS = {
     "bsim_sim_stream": (P, [P]),
     "bsim_sim_destroy": (None, [P]),
     "bsim_net_gen_standard": (C.c_int64, [P, P]),
+    "bsim_net_kinds": (C.c_int32, [P, P]),
     "bsim_net_export_neurons": (C.c_int32, [P, P, P, P, P]),
     "bsim_net_export_synapses": (C.c_int64, [P, C.c_int64, C.c_int64, P, P, P, P, P]),
     "bsim_net_lif_constants": (C.c_int32, [P, C.c_float, P, P]),
@@ -347,6 +351,16 @@ class Network:
         return dict(ntiles=nt.value, s=s.value, dmax=dm.value, tile_syn=ts, syn_tgt=tg, syn_w=w)
 
 
+def run_cycles(T, dt) -> int:
+    """round(T/dt) as SingleGPUSimulator::run computes it
+    (src/sim/SingleGPUSimulator.cu:34): the quotient in float, then C's
+    round(), which takes halves away from zero (numpy's round takes them to
+    even).  The float32 quotient plus 0.5 is exact in double."""
+    q = float(np.float32(T) / np.float32(dt))
+    r = math.floor(abs(q) + 0.5)
+    return int(r if q >= 0 else -r)
+
+
 class Simulation:
     """Device simulation (bsim_sim): SGSim's run loop on one GPU."""
 
@@ -388,7 +402,7 @@ class Simulation:
     def run(self, T):
         """SingleGPUSimulator::run(T): round(T/dt) cycles (computed in float,
         src/sim/SingleGPUSimulator.cu:34), then the last cycle's phase (ii)."""
-        cycles = int(np.round(np.float32(T) / self.dt))
+        cycles = run_cycles(T, self.dt)
         self.step(cycles)
         self.finish()
         self.sync()
@@ -419,6 +433,22 @@ class Simulation:
         if r < 0:
             _err(r, "bsim_sim_read_trace")
         return v
+
+    def enable_state_trace(self):
+        """i_E / i_I after the update and the E / I input it consumed, per
+        traced neuron and cycle (bsim_sim_enable_state_trace)."""
+        r = lib().bsim_sim_enable_state_trace(self._h)
+        if r < 0:
+            _err(r, "bsim_sim_enable_state_trace")
+
+    def state_trace(self):
+        steps = min(self.cycle, self.rec_steps)
+        out = {k: np.zeros((steps, self.n_rec), np.float32) for k in ("ie", "ii", "in_e", "in_i")}
+        r = lib().bsim_sim_read_state_trace(self._h, _p(out["ie"]), _p(out["ii"]),
+                                            _p(out["in_e"]), _p(out["in_i"]))
+        if r < 0:
+            _err(r, "bsim_sim_read_state_trace")
+        return out
 
     def state(self):
         n = self.n

@@ -32,7 +32,7 @@ def flags():
 
 
 def build(verbose: bool = False) -> str:
-    objs = []
+    objs, cmds = [], []
     bdir = os.path.join(HERE, "_obj")
     os.makedirs(bdir, exist_ok=True)
     for s in SOURCES:
@@ -44,8 +44,14 @@ def build(verbose: bool = False) -> str:
             cmd.insert(2, "hip")
         if verbose:
             print(" ".join(cmd), flush=True)
-        subprocess.run(cmd, check=True)
+        cmds.append(cmd)
         objs.append(obj)
+    # the translation units compile independently, in parallel
+    from concurrent.futures import ThreadPoolExecutor
+    with ThreadPoolExecutor(max_workers=len(cmds)) as ex:
+        for r in list(ex.map(lambda c: subprocess.run(c), cmds)):
+            if r.returncode != 0:
+                raise subprocess.CalledProcessError(r.returncode, r.args)
     tmp = LIB + ".tmp"
     cmd = [hipcc(), "--offload-arch=gfx950", "-shared", "-fopenmp", "-o", tmp, *objs]
     if verbose:

@@ -105,7 +105,7 @@ int32_t bsim_net_add_array(bsim_network *net, int32_t n, const int64_t *offsets,
                 row[10] = taus[2 * i];
                 row[11] = taus[2 * i + 1];
             } else {
-                row[10] = 1e-3f;   // CompositeNeuron's default taus (CompositeNeuron.h:160)
+                row[10] = 1e-3f;   // CompositeNeuron's default taus (CompositeNeuron.h:21)
                 row[11] = 1e-3f;
             }
             h.lif.insert(h.lif.end(), row, row + bsim::kLifParams);
@@ -172,7 +172,7 @@ int32_t bsim_net_add_poisson(bsim_network *net, int32_t n, const float *rate, co
         for (int32_t i = 0; i < n; i++) {
             float row[bsim::kLifParams] = {0};
             row[10] = taus ? taus[2 * i] : 1e-3f;       // CompositeNeuron's default taus
-            row[11] = taus ? taus[2 * i + 1] : 1e-3f;   // (CompositeNeuron.h:160)
+            row[11] = taus ? taus[2 * i + 1] : 1e-3f;   // (CompositeNeuron.h:21)
             h.lif.insert(h.lif.end(), row, row + bsim::kLifParams);
             h.pois.push_back({first + i, rate[i], start[i], start[i] + duration[i]});
         }
@@ -665,6 +665,61 @@ int32_t bsim_sim_read_trace(bsim_sim *sim, float *v)
     for (int32_t t = 0; t < steps; t++)
         for (int32_t r = 0; r < sim->n_rec; r++)
             v[(size_t)t * sim->n_rec + r] = cols[(size_t)t * ncol + sim->rec_col[r]];
+    return steps;
+}
+
+int32_t bsim_sim_enable_state_trace(bsim_sim *sim)
+{
+    if (!sim) return fail(BSIM_E_ARG, "bad argument");
+    if (!sim->d.rec_v) return fail(BSIM_E_STATE, "no neuron is traced");
+    if (sim->d.rec_ie) return 0;
+    if (sim->cycle > 0) return fail(BSIM_E_STATE, "state traces must be enabled before the first cycle");
+    hipError_t e = hipSetDevice(sim->device);
+    if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    const size_t vb = (size_t)sim->rec_steps * (size_t)sim->d.n_rec;
+    float *buf[4];
+    for (int i = 0; i < 4; i++) {
+        buf[i] = dev_alloc<float>(sim, vb, e);
+        if (!buf[i]) return fail(e == hipErrorOutOfMemory ? BSIM_E_NOMEM : BSIM_E_HIP,
+                                 std::string("state trace: ") + hipGetErrorString(e));
+        e = hipMemsetAsync(buf[i], 0, vb * sizeof(float), sim->stream);
+        if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("state trace: ") + hipGetErrorString(e));
+    }
+    sim->d.rec_ie = buf[0];
+    sim->d.rec_ii = buf[1];
+    sim->d.rec_in_e = buf[2];
+    sim->d.rec_in_i = buf[3];
+    /* the captured launches hold the kernel arguments: capture again */
+    if (sim->gexec) {
+        if ((e = hipStreamSynchronize(sim->stream)) != hipSuccess)
+            return fail(BSIM_E_HIP, std::string("state trace: ") + hipGetErrorString(e));
+        hipGraphExecDestroy(sim->gexec);
+        hipGraphDestroy(sim->graph);
+        sim->gexec = nullptr;
+        sim->graph = nullptr;
+    }
+    return 0;
+}
+
+int32_t bsim_sim_read_state_trace(bsim_sim *sim, float *ie, float *ii, float *in_e, float *in_i)
+{
+    if (!sim) return fail(BSIM_E_ARG, "bad argument");
+    if (!sim->d.rec_ie) return fail(BSIM_E_STATE, "state traces not enabled (bsim_sim_enable_state_trace)");
+    hipError_t e = hipSetDevice(sim->device);
+    if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    const int32_t steps = sim->cycle < sim->rec_steps ? sim->cycle : sim->rec_steps;
+    const int32_t ncol = sim->d.n_rec;
+    float *out[4] = {ie, ii, in_e, in_i};
+    const float *src[4] = {sim->d.rec_ie, sim->d.rec_ii, sim->d.rec_in_e, sim->d.rec_in_i};
+    std::vector<float> cols((size_t)steps * (size_t)ncol);
+    for (int i = 0; i < 4; i++) {
+        if (!out[i]) continue;
+        e = fetch(sim, cols.data(), src[i], cols.size());
+        if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("state trace copy: ") + hipGetErrorString(e));
+        for (int32_t t = 0; t < steps; t++)
+            for (int32_t r = 0; r < sim->n_rec; r++)
+                out[i][(size_t)t * sim->n_rec + r] = cols[(size_t)t * ncol + sim->rec_col[r]];
+    }
     return steps;
 }
 

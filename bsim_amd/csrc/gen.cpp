@@ -433,6 +433,16 @@ extern "C" int32_t bsim_net_export_neurons(const bsim_network *netp, int8_t *kin
     return h.n();
 }
 
+extern "C" int32_t bsim_net_kinds(const bsim_network *netp, int8_t *kind)
+{
+    if (!netp || !kind) {
+        g_bsim_err = "bad argument";
+        return BSIM_E_ARG;
+    }
+    std::copy(netp->net.kind.begin(), netp->net.kind.end(), kind);
+    return netp->net.n();
+}
+
 extern "C" int64_t bsim_net_export_synapses(const bsim_network *netp, int64_t start, int64_t count,
                                             int32_t *pre, int32_t *post, float *w, float *d,
                                             int8_t *stype)

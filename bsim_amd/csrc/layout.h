@@ -143,6 +143,11 @@ typedef struct bsim_dev {
     int32_t pad1;
     const int32_t *rec_index;   /* [n]: trace column of a neuron or -1 (NULL: none) */
     float *rec_v;               /* [rec_steps][n_rec] */
+    /* state traces of the traced neurons (NULL: off), [rec_steps][n_rec] each:
+     * i_E and i_I after the update, and the E / I synaptic input the update
+     * consumed (the reference's -DLOG_DATA files g_ie.data, g_ii.data,
+     * g_input_e.data, g_input_i.data, src/sim/SingleGPUSimulator.cu:150-216) */
+    float *rec_ie, *rec_ii, *rec_in_e, *rec_in_i;
     /* input logs (NULL: off).  in_sum[n]: a LIF neuron's E + I input summed
      * over the cycles it neither was refractory nor fired (the reference GPU
      * simulator's gXInput, src/neuron/lif/lif.cu:116); slot_count[n]: the

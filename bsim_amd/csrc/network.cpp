@@ -19,12 +19,15 @@
 
 namespace bsim {
 
-static const float kZero = 1e-10f;  // `const real ZERO = 1e-10;` (src/utils/constant.h:19)
+static const float kZero = 1e-10f;  // `const real ZERO = 1e-10;` (src/utils/constant.h:20)
 
 /* Two pres with synapses into a tile share a source range when fewer than
  * this many pres without synapses into it lie between them. */
 static const int32_t kRangeGap = 256;
 static const int32_t kGroupTiles = 32;   /* tiles per target group */
+/* Runs of ascending (pre, delay) order per target, in creation order, that a
+ * tile's layout keeps apart (segments, see build_layout). */
+static const int32_t kMaxSeg = 32;
 
 void lif_constants(const float *p, float dt, float out[9], int32_t *refrac_time)
 {
@@ -179,7 +182,7 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
             return -1;
         }
     }
-    /* Poisson sources: PoissonNeuron::reset (src/neuron/poisson/PoissonNeuron.cpp:37-45)
+    /* Poisson sources: PoissonNeuron::reset (src/neuron/poisson/PoissonNeuron.cpp:36-45)
      * -- first firing cycle int(startTime / dt) + X, end cycle int(endTime / dt);
      * the first draw uses t = 0xFFFFFFFF, a cycle the simulation never reaches.
      * One table per distinct rate. */
@@ -301,46 +304,64 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
     int fail = 0;
     int64_t first_unordered = S;
 
-    /* Ordering property: the GPU sums a target's inputs in (pre, delay,
-     * creation) order; the reference sums them in creation order
-     * (Network.cpp:324-327).  They agree when each target's incoming
-     * synapses, taken in creation order, have non-decreasing (pre, delay).
-     * Checked per tile below (a tile's bucket keeps creation order); the
-     * first offending synapse over all tiles is reported. */
+    /* Summation order.  The reference sums a target's inputs of a cycle in
+     * synapse creation order (Network.cpp:324-327).  Each target's incoming
+     * synapses, taken in creation order, are cut into runs at every step where
+     * (pre, delay) descends; the run index of a synapse is its segment (0 for
+     * a target whose inputs were created pre by pre; 1 from the second
+     * `connect` call on when a target is fed first by a later population and
+     * then by an earlier one, as test/gpu/standard_test.cpp:116-121 does).
+     * The tile's synapses are sorted by (segment, pre, delay, part, creation)
+     * and its source ranges are formed per segment, so the stream order of a
+     * target's inputs -- segment by segment, (pre, delay) ascending inside
+     * one -- is its creation order, for any network.  A segment of a tile
+     * costs its own source ranges and row-table vectors, so a tile whose
+     * targets have more than kMaxSeg segments is rejected (the first synapse
+     * that opens segment kMaxSeg is reported). */
 #pragma omp parallel for schedule(dynamic, 1) reduction(min : first_unordered)
     for (int32_t t = 0; t < NT; t++) {
         const int64_t lo = L.tile_syn[t], hi = L.tile_syn[t + 1], m = hi - lo;
-        /* row key: pre (31 bits) | delay (6 bits) | part (2 bits); ties
-         * keep the creation rank */
+        /* row key: segment (5 bits) | pre (31 bits) | delay (6 bits) |
+         * part (2 bits); ties keep the creation rank */
         struct Ent { uint64_t key; uint32_t rank; };
         std::vector<Ent> ent(m);
         std::vector<int64_t> last(tile, -1);
+        std::vector<uint8_t> seg(tile, 0);
         for (int64_t e = 0; e < m; e++) {
             const int64_t k = order[lo + e];
             const int32_t lt = net.post[k] - t * tile;
             const int64_t okey = ((int64_t)net.pre[k] << 8) | ds[k];
-            if (okey < last[lt] && k < first_unordered) first_unordered = k;
+            if (okey < last[lt]) {
+                if (seg[lt] + 1 >= kMaxSeg) {
+                    if (k < first_unordered) first_unordered = k;
+                } else {
+                    seg[lt]++;
+                }
+            }
             last[lt] = okey;
             const uint32_t q = (uint32_t)(lt / qw);
-            ent[e].key = ((uint64_t)(uint32_t)net.pre[k] << 8) | ((uint64_t)ds[k] << 2) | q;
+            ent[e].key = ((uint64_t)seg[lt] << 39) | ((uint64_t)(uint32_t)net.pre[k] << 8) |
+                         ((uint64_t)ds[k] << 2) | q;
             ent[e].rank = (uint32_t)e;
         }
-        std::sort(ent.begin(), ent.end(), [](const Ent &a, const Ent &b) {
-            return a.key < b.key || (a.key == b.key && a.rank < b.rank);
+        std::sort(ent.begin(), ent.end(), [](const Ent &x, const Ent &y) {
+            return x.key < y.key || (x.key == y.key && x.rank < y.rank);
         });
         for (int64_t e = 0; e < m; e++) {
             const int64_t k = order[lo + ent[e].rank];
             L.syn_tgt[lo + e] = (uint16_t)(net.post[k] - t * tile);
             L.syn_w[lo + e] = w[k];
         }
-        /* source ranges: runs of pres whose gaps are at most kRangeGap */
+        /* source ranges, per segment: runs of pres whose gaps are at most
+         * kRangeGap */
         int64_t e = 0;
         while (e < m) {
-            const int32_t p0 = (int32_t)(ent[e].key >> 8);
+            const uint64_t sg = ent[e].key >> 39;
+            const int32_t p0 = (int32_t)((ent[e].key >> 8) & 0x7FFFFFFFu);
             int64_t f = e;
             int32_t plast = p0;
-            while (f < m) {
-                const int32_t pf = (int32_t)(ent[f].key >> 8);
+            while (f < m && (ent[f].key >> 39) == sg) {
+                const int32_t pf = (int32_t)((ent[f].key >> 8) & 0x7FFFFFFFu);
                 if (pf - plast > kRangeGap || pf - p0 >= BSIM_MAX_RANGE) break;
                 plast = pf;
                 f++;
@@ -361,7 +382,8 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
             for (int32_t p = r.lo; p < r.hi; p++) {
                 for (int32_t d = 1; d <= D; d++) {
                     for (int32_t q = 0; q < NW; q++) {
-                        const uint64_t key = ((uint64_t)(uint32_t)p << 8) | ((uint64_t)d << 2) | (uint64_t)q;
+                        const uint64_t key = (sg << 39) | ((uint64_t)(uint32_t)p << 8) |
+                                             ((uint64_t)d << 2) | (uint64_t)q;
                         toff[t][x++] = (uint32_t)pos;
                         while (pos < f && ent[pos].key == key) pos++;
                     }
@@ -376,10 +398,10 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
     if (first_unordered < S) {
         const int64_t k = first_unordered;
         err = "synapse " + std::to_string(k) + " (" + std::to_string(net.pre[k]) + " -> " +
-              std::to_string(net.post[k]) +
-              ") was created after a synapse into the same target from a later "
-              "(pre, delay); the GPU step requires each target's inputs to be created "
-              "in ascending (pre, delay) order";
+              std::to_string(net.post[k]) + ") would open run " + std::to_string(kMaxSeg + 1) +
+              " of ascending (pre, delay) order among its target's inputs in creation "
+              "order; the GPU layout keeps at most " + std::to_string(kMaxSeg) +
+              " such runs per target (each costs its own row tables)";
         return -5;
     }
     if (fail) {

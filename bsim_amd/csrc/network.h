@@ -78,13 +78,13 @@ struct Layout {
     std::vector<uint32_t> chunk_wrow;
 };
 
-/* LIFENeuron::reset (src/neuron/lif/LIFENeuron.cpp:123-167) for one neuron,
+/* LIFENeuron::reset (src/neuron/lif/LIFENeuron.cpp:68-112) for one neuron,
  * in float, in the reference's operation order.  out: Cm CE CI C_E C_I v_tmp
  * v_thresh v_reset v_init. */
 void lif_constants(const float *p, float dt, float out[9], int32_t *refrac_time);
 
-/* StaticSynapse::reset (src/synapse/static/StaticSynapse.cpp:81-92) with the
- * tau picked as CompositeNeuron::createSynapse does (CompositeNeuron.h:200-218).
+/* StaticSynapse::reset (src/synapse/static/StaticSynapse.cpp:43-54) with the
+ * tau picked as CompositeNeuron::createSynapse does (CompositeNeuron.h:62-79).
  * Returns false where the reference asserts (non-integral delay, zero tau). */
 bool synapse_reset(float w, float delay, float tau, float dt, float *w_out, int32_t *ds);
 

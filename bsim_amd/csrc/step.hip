@@ -7,8 +7,8 @@
  *   (ii) phase (ii) of cycle t-1: every synapse into the tile whose pre fired
  *        d cycles earlier adds its weight to the target's E or I accumulator,
  *        in the order the reference's CPU simulator applies them
- *        (src/net/Network.cpp:324-327, src/neuron/lif/LIFENeuron.cpp:107-116);
- *   (i)  the LIF_brian update of cycle t (src/neuron/lif/LIFENeuron.cpp:169-215),
+ *        (src/net/Network.cpp:324-327, src/neuron/lif/LIFENeuron.cpp:52-61);
+ *   (i)  the LIF_brian update of cycle t (src/neuron/lif/LIFENeuron.cpp:114-160),
  *        which consumes exactly those accumulators, the spike history update
  *        and the ordered compaction of the tile's fired / active neurons into
  *        64-bit words by wavefront ballot.
@@ -651,7 +651,16 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
                 if (fired) d.fire_count[nid] += 1;
                 if (rec && d.rec_index) {
                     const int32_t ri = d.rec_index[nid];
-                    if (ri >= 0) d.rec_v[(int64_t)(t - d.rec_t0) * d.n_rec + ri] = d.v[nid];
+                    if (ri >= 0) {
+                        const int64_t o = (int64_t)(t - d.rec_t0) * d.n_rec + ri;
+                        d.rec_v[o] = d.v[nid];
+                        if (d.rec_ie) {
+                            d.rec_ie[o] = d.ie[nid];
+                            d.rec_ii[o] = d.ii[nid];
+                            d.rec_in_e[o] = kind == BSIM_KIND_LIF ? acc.e[j] : 0.0f;
+                            d.rec_in_i[o] = kind == BSIM_KIND_LIF ? acc.i[j] : 0.0f;
+                        }
+                    }
                 }
             }
             const uint64_t fm = ballot64(fired);

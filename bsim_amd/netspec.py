@@ -10,16 +10,16 @@ to float32 once (``(float)x``), exactly as passing a double literal to a
 Per-neuron LIF row (``LIF_NPARAM`` values), the arguments of
 ``LIF_brian(LIFENeuron(v_init, v_rest, v_reset, cm, tau_m, tau_refrac,
 tau_syn_E, tau_syn_I, v_thresh, i_offset), tau_syn_E, tau_syn_I)``
-(src/neuron/lif/LIFENeuron.h:9, src/base/CompositeNeuron.h:160):
+(src/neuron/lif/LIFENeuron.h:9, src/base/CompositeNeuron.h:21):
 
     0 v_init  1 v_rest  2 v_reset  3 cm  4 tau_m  5 tau_refrac
     6 tau_syn_E  7 tau_syn_I  8 v_thresh  9 i_offset
     10 composite tau_syn_E  11 composite tau_syn_I
 
 Neuron kinds: ``LIF`` (LIF_brian), ``ARRAY`` (Array_spikes: fires at the
-listed cycles, src/neuron/array/ArrayNeuron.cpp:115-128), ``BERNOULLI``
+listed cycles, src/neuron/array/ArrayNeuron.cpp:67-80), ``BERNOULLI``
 (a seeded per-cycle Bernoulli source) and ``POISSON`` (Poisson_spikes with the
-reference's CPU semantics, src/neuron/poisson/PoissonNeuron.cpp:37-63: first
+reference's CPU semantics, src/neuron/poisson/PoissonNeuron.cpp:36-63: first
 spike at int(start/dt) + X, then every 1 + X cycles before int(end/dt), X ~
 Poisson(rate) drawn from seeded hashes instead of the reference's
 clock-seeded engine, PoissonNeuron.cpp:25-26).
@@ -339,11 +339,17 @@ def standard(npop=4, n=1000, fanout=100, w0=1.0, inh_frac=0.25, dmax=6, remote_f
 C1 = dict(npop=64, n=10000, fanout=1000, w0=0.45, inh_frac=0.25, dmax=6, remote_frac=0.2,
           drive_rate_hz=20.0, n_drive=10000, drive_w=1.0, drive_fanout=100, seed=1,
           i_offset=(14.5, 16.5))
-# High-activity stress workload: 100 x 10,000 LIF, ~2,000 out-synapses per
-# neuron, w0 x 1.3, 20 % inhibitory, delays 1..50 steps, no refractory period.
-C2 = dict(npop=100, n=10000, fanout=2000, w0=0.45 * 1.3, inh_frac=0.2, dmax=50,
+# High-activity stress workload at ~2,000 Hz: 100 x 10,000 LIF, 2,200
+# out-synapses per neuron (2.19e9 synapses: past 2^31, so that every 64-bit
+# offset of the layout and the kernels is in use), w0 x 1.3, 20 % inhibitory,
+# delays 1..50 steps.
+# The refractory period of 0.35 ms (refrac_time = int(0.35e-3 / 1e-4) = 3
+# steps) makes a saturated neuron fire every fifth step, i.e. 2,000 Hz; the
+# chain's populations saturate one after another within the first ~1,000
+# cycles (without a refractory period they fire every other step, 5,000 Hz).
+C2 = dict(npop=100, n=10000, fanout=2200, w0=0.45 * 1.3, inh_frac=0.2, dmax=50,
           remote_frac=0.2, drive_rate_hz=20.0, n_drive=10000, drive_w=1.0, drive_fanout=100,
-          seed=2, i_offset=(14.5, 16.5), tau_refrac=0.0)
+          seed=2, i_offset=(14.5, 16.5), tau_refrac=0.35e-3)
 
 
 def c1(**over):
@@ -399,3 +405,95 @@ def random_net(seed, n_lif=40, n_arr=5, n_bern=3, p_conn=0.2, dmax=5, allow_dup=
 
 def b_kind(b: Builder):
     return np.concatenate(b.kinds)
+
+
+def standard_test(depth=8, n=100, w=(0.7, 0.5, 0.6, 0.3), who=3) -> NetSpec:
+    """The reference's benchmark program test/gpu/standard_test.cpp:55-138 at
+    a chosen depth and population size, with its own neuron parameters
+    (v_thresh -54 mV, v_reset -60 mV, c_m 0.2 nF, v_rest -74 mV, tau_m 10 ms,
+    no refractory period, tau_syn 2 / 0.6 ms, i_offset 2 nA on g[1] only),
+    weights (1e-9 * w_k / n, computed in float as there), 0.1 ms delays and
+    call order:
+
+    * g[1..depth] created first; for i = 2..depth, g[i-1] -> g[i] all-to-all
+      and, when i % who == 0, g[1] -> g[i] (so those targets are fed by a
+      later population first, then by an earlier one);
+    * then for i = 1, 5, 9, ... (i + 1 <= depth) a side population p[i]:
+      g[i] -> p[i] (w3) and p[i] -> g[i+1] (-w4, Inhibitory).
+
+    README.md:87-102 names w = (0.7, 0.5, 0.6, 0.3) with who = 6 the 100 Hz
+    regime of depth x 1000 neurons."""
+    f = np.float32
+    fthreshold, freset, c_m, v_rest = f(-54e-3), f(-60e-3), f(0.2e-9), f(-74e-3)
+    tau_syn_e, tau_syn_i, tau_m, i_off, frefr, fv = f(2e-3), f(0.6e-3), f(10e-3), f(2e-9), f(0), \
+        f(-74e-3)
+
+    def row(ioff):
+        return lif_row(fv, v_rest, freset, c_m, tau_m, frefr, tau_syn_e, tau_syn_i, fthreshold,
+                       ioff, tau_syn_e, tau_syn_i)
+
+    w1, w2, w3, w4 = (f(x) for x in w)
+    nn = n * n
+    weight6 = np.full(nn, f(1e-9) * w1 / f(n), np.float32)
+    weight6_30 = np.full(nn, f(1e-9) * w2 / f(n), np.float32)
+    weight5 = np.full(nn, f(1e-9) * w3 / f(n), np.float32)
+    weight3 = np.full(nn, f(-1e-9) * w4 / f(n), np.float32)
+    delay = np.full(nn, f(0.1e-3), np.float32)
+    b = Builder()
+    g = [None, b.population(np.tile(row(i_off), (n, 1)))]
+    for _ in range(2, depth + 1):
+        g.append(b.population(np.tile(row(f(0)), (n, 1))))
+    for i in range(2, depth + 1):
+        b.connect_all_to_all(g[i - 1], g[i], weight6, delay)
+        if i % who == 0:
+            b.connect_all_to_all(g[1], g[i], weight6_30, delay)
+    i = 1
+    while i + 1 <= depth:
+        p = b.population(np.tile(row(f(0)), (n, 1)))
+        b.connect_all_to_all(g[i], p, weight5, delay)
+        b.connect_all_to_all(p, g[i + 1], weight3, delay, np.ones(nn, np.int8))
+        i += 4
+    return b.build()
+
+
+def nonint_net(seed=1, n_lif=80, n_arr=12, p_conn=0.2, dmax=6, steps=300) -> NetSpec:
+    """Small network whose delays are not whole steps, to exercise
+    StaticSynapse::reset's rescale w *= expf(-(delay - dt * round(delay /
+    dt)) / tau) (src/synapse/static/StaticSynapse.cpp:43-54) and the tau
+    choice of CompositeNeuron::createSynapse (src/base/CompositeNeuron.h:
+    62-79): an explicit per-synapse tau on a third of the synapses, else the
+    destination's composite tau_syn_E or tau_syn_I by synapse type.  Types
+    are drawn independently of the weight's sign (the type only picks the
+    tau; the sign picks the E or I accumulator), composite taus differ from
+    the neuron's own and between E and I, and delay fractions include
+    exact halves (round() takes them away from zero).  Creation order is
+    pre-major with ascending delays."""
+    rng = np.random.Generator(np.random.PCG64(seed))
+    b = Builder()
+    sp = [np.sort(rng.choice(steps, size=rng.integers(steps // 10, steps // 4 + 1),
+                             replace=False)) for _ in range(n_arr)]
+    b.population(np.zeros((n_arr, LIF_NPARAM)), kind=ARRAY, spikes=sp)
+    rows = np.tile(lif_row(0.0, -0.5, 0.0, 0.02, 0.02, 5e-4, 5e-3, 8e-3, 1.0, 0.0,
+                           3e-3, 12e-3), (n_lif, 1))
+    rows[:, 9] = rng.uniform(0.6, 1.5, n_lif)
+    rows[n_lif // 2:, 10] = 7e-3       # a second population-like block with other taus
+    rows[n_lif // 2:, 11] = 1.5e-3
+    b.population(rows)
+    n = b.n
+    pre, post = np.nonzero(rng.random((n, n)) < p_conn)
+    keep = post >= n_arr
+    pre, post = pre[keep], post[keep]
+    m = len(pre)
+    k = rng.integers(1, dmax + 1, m)
+    frac = rng.choice(np.array([0.0, 0.5, -0.3, 0.27, 0.41, -0.45, 0.12]), m)
+    frac[k == 1] = np.abs(frac[k == 1])          # keep round(delay / dt) >= 1
+    order = np.lexsort((k + frac, post, pre))
+    pre, post, k, frac = pre[order], post[order], k[order], frac[order]
+    w = np.where(rng.random(m) < 0.3, -rng.uniform(0, 0.5, m), rng.uniform(0, 0.4, m))
+    stype = (rng.random(m) < 0.4).astype(np.int8)
+    tau = np.where(rng.random(m) < 1 / 3, rng.uniform(1e-3, 10e-3, m), 0.0)
+    d = ((k + frac) * DT).astype(np.float32)
+    b.connect(pre, post, w.astype(np.float32), d, stype, tau.astype(np.float32))
+    spec = b.build()
+    spec.seed = seed
+    return spec

@@ -30,6 +30,7 @@
  *   SGSim(Network*, dt), run(T), run(T, FireInfo&) src/sim/SingleGPUSimulator.h:12-16,
  *                                                  src/base/SimulatorBase.cpp:29
  *   FireInfo, ArrayInfo                            src/base/SimInfo.h:26-31
+ *   getConstArray / getRandomArray                 src/utils/utils.h:34-58
  *
  * Differences a user can observe:
  *   - Neuron ids are creation order over the whole network (the reference
@@ -40,11 +41,14 @@
  *     returns a handle valid until the next connect.
  *   - Errors (an unsupported network, a HIP failure) throw bsim_error rather
  *     than exiting the process.
- *   - STSim is the same simulator as SGSim (the CPU path is not shipped).
+ *   - STSim (the reference's CPU simulator) is not provided: constructing
+ *     one fails to compile with a message (use SGSim).
  *   - Output files: GSim.info always; GSim.log (fired ids per cycle, ascending)
  *     and g_v.data (V of the monitored neurons per cycle, "%.10lf \t") when
- *     enabled with setLog / monitor (the reference writes them under
- *     -DLOG_DATA, src/sim/SingleGPUSimulator.cu:176-232).  FireInfo "count",
+ *     enabled with setLog / monitor; setLogData(true) writes what the
+ *     reference writes under -DLOG_DATA (src/sim/SingleGPUSimulator.cu:
+ *     150-232): GSim.log and, for every LIF neuron, g_v.data, g_ie.data,
+ *     g_ii.data, g_input_e.data and g_input_i.data.  FireInfo "count",
  *     "Y" (summed LIF input) and "X" (delay slots visited) are filled as the
  *     reference GPU simulator fills them (src/sim/SingleGPUSimulator.cu:263-296);
  *     run(T) writes GFire.count (src/base/SimulatorBase.cpp:29-37).
@@ -57,6 +61,7 @@
 #include <stdio.h>
 #include <stdlib.h>
 #include <sys/time.h>
+#include <time.h>
 
 #include <map>
 #include <stdexcept>
@@ -79,6 +84,33 @@ class bsim_error : public std::runtime_error {
 public:
     explicit bsim_error(const std::string &what) : std::runtime_error(what) {}
 };
+
+/* Array helpers of the reference's utils (src/utils/utils.h:34-58), which its
+ * programs use to build weight / delay / type arrays for connect.  The caller
+ * owns the result (delete[]). */
+template <typename T>
+T *getConstArray(T value, int size)
+{
+    T *a = new T[size > 0 ? size : 1];
+    for (int i = 0; i < size; i++) a[i] = value;
+    return a;
+}
+
+/* Uniform values in [0, range] from rand(), seeded once from the clock as the
+ * reference does (its realRandom, src/utils/utils.cpp:9-13). */
+template <typename T>
+T *getRandomArray(T range, int size)
+{
+    static bool seeded = false;
+    if (!seeded) {
+        srand((unsigned)time(NULL));
+        seeded = true;
+    }
+    T *a = new T[size > 0 ? size : 1];
+    for (int i = 0; i < size; i++)
+        a[i] = static_cast<T>((double)rand() / (double)RAND_MAX * static_cast<double>(range));
+    return a;
+}
 
 namespace bsim_cpp {
 inline void check(int64_t rc, const char *what)
@@ -432,13 +464,20 @@ private:
 class SingleGPUSimulator {
 public:
     SingleGPUSimulator(Network *network, real dt)
-        : network_(network), dt_(dt), device_(0), log_raster_(false) {}
+        : network_(network), dt_(dt), device_(0), log_raster_(false), log_data_(false) {}
     virtual ~SingleGPUSimulator() {}
 
     /* Extensions: GSim.log raster and g_v.data trace of the given neurons. */
     void setLog(bool raster) { log_raster_ = raster; }
     void monitor(const std::vector<int> &gids) { monitor_ = gids; }
     void setDevice(int device) { device_ = device; }
+    /* The output of the reference built with -DLOG_DATA
+     * (src/sim/SingleGPUSimulator.cu:150-232): GSim.log, and per cycle one
+     * line of every LIF_brian neuron's V (g_v.data), i_E and i_I after the
+     * update (g_ie.data, g_ii.data) and the E / I synaptic input the update
+     * consumed (g_input_e.data, g_input_i.data), "%.10lf \t" per neuron.
+     * Replaces monitor(). */
+    void setLogData(bool on) { log_data_ = on; }
 
     /* SimulatorBase::run (src/base/SimulatorBase.cpp:29-37): runs with a
      * "count" log and writes it to GFire.count ("%d " per neuron, then a
@@ -467,10 +506,20 @@ public:
     {
         const int cycles = (int)roundf(time / dt_);
         const bsim_network *net = network_->handle();
+        if (log_data_) {
+            /* every LIF neuron, in id order */
+            const int32_t n = bsim_net_num_neurons(net);
+            std::vector<int8_t> kind((size_t)(n > 0 ? n : 1));
+            bsim_cpp::check(bsim_net_kinds(net, kind.data()), "SGSim");
+            monitor_.clear();
+            for (int32_t i = 0; i < n; i++)
+                if (kind[i] == 0) monitor_.push_back(i);
+        }
+        const bool raster = log_raster_ || log_data_;
         bsim_sim_config cfg;
         cfg.device = device_;
         cfg.tile = 0;
-        cfg.rec_steps = (log_raster_ || !monitor_.empty()) ? cycles : 0;
+        cfg.rec_steps = (raster || !monitor_.empty()) ? cycles : 0;
         cfg.n_rec = (int32_t)monitor_.size();
         cfg.rec_ids = monitor_.empty() ? NULL : monitor_.data();
         bsim_sim *sim = bsim_sim_create(net, dt_, &cfg);
@@ -479,6 +528,8 @@ public:
             bsim_sim *s;
             ~Guard() { bsim_sim_destroy(s); }
         } guard = {sim};
+        if (log_data_ && !monitor_.empty())
+            bsim_cpp::check(bsim_sim_enable_state_trace(sim), "SGSim state trace");
         bsim_sim_info info;
         bsim_cpp::check(bsim_sim_get_info(sim, &info), "SGSim");
         const bool want_y = log.find("Y") != log.end(), want_x = log.find("X") != log.end();
@@ -504,8 +555,9 @@ public:
         fprintf(inf, "Synaptic events: %lld\n", (long long)bsim_sim_events(sim));
         fclose(inf);
 
-        if (log_raster_) write_raster(sim, cycles);
+        if (raster) write_raster(sim, cycles);
         if (!monitor_.empty()) write_trace(sim, cycles);
+        if (log_data_ && !monitor_.empty()) write_state_traces(sim, cycles);
 
         if (log.find("count") != log.end()) {
             int *count = (int *)malloc(sizeof(int) * (size_t)(info.n > 0 ? info.n : 1));
@@ -566,14 +618,50 @@ private:
         fclose(f);
     }
 
+    void write_state_traces(bsim_sim *sim, int cycles)
+    {
+        const size_t m = monitor_.size(), sz = (size_t)cycles * m + 1;
+        std::vector<float> a[4] = {std::vector<float>(sz), std::vector<float>(sz),
+                                   std::vector<float>(sz), std::vector<float>(sz)};
+        bsim_cpp::check(bsim_sim_read_state_trace(sim, a[0].data(), a[1].data(), a[2].data(),
+                                                  a[3].data()),
+                        "SGSim state trace");
+        const char *names[4] = {"g_ie.data", "g_ii.data", "g_input_e.data", "g_input_i.data"};
+        for (int k = 0; k < 4; k++) {
+            FILE *f = fopen(names[k], "w+");
+            if (!f) throw bsim_error(std::string("Open file ") + names[k] + " failed");
+            for (int c = 0; c < cycles; c++) {
+                for (size_t i = 0; i < m; i++) fprintf(f, "%.10lf \t", (double)a[k][(size_t)c * m + i]);
+                fprintf(f, "\n");
+            }
+            fclose(f);
+        }
+    }
+
     Network *network_;
     real dt_;
     int device_;
     bool log_raster_;
+    bool log_data_;
     std::vector<int> monitor_;
 };
 
 typedef SingleGPUSimulator SGSim;
-typedef SingleGPUSimulator STSim;
+
+/* The reference's single-threaded CPU simulator (src/sim/
+ * SingleThreadSimulator.cpp) is not part of this library: the step runs on
+ * the GPU only.  Constructing an STSim is a compile-time error, so that a
+ * program that compares STSim with SGSim (as test/gpu/gpusim_unit_test.cpp
+ * does) cannot silently compare the GPU with itself. */
+class STSim {
+public:
+    template <class... A>
+    STSim(A &&...)
+    {
+        static_assert(sizeof...(A) == 0 && sizeof...(A) != 0,
+                      "STSim (the reference's CPU simulator) is not provided by bsim_amd; "
+                      "use SGSim, the GPU simulator");
+    }
+};
 
 #endif

@@ -28,12 +28,12 @@
  *                          copyNetworkToGPU + alloc_buffers + getBlockSize
  *                          (src/net/Network.cpp:286-306, src/net/BuildNetwork.cpp:36-205,
  *                          src/gpu_utils/gpu_net_copy.cu:8-109, src/gpu_utils/GBuffers.cu:9-56,
- *                          src/gpu_utils/gpu_func.cu:195-232; per type cudaAllocLIFE /
- *                          cudaAllocStatic, src/utils/macros.h:37-58)
+ *                          src/gpu_utils/gpu_func.cu:90-127; per type cudaAllocLIFE /
+ *                          cudaAllocStatic, src/utils/macros.h:11-32)
  *      bsim_sim_step    <- one cycle of SingleGPUSimulator::run's loop:
  *                          cudaUpdateLIFE + cudaUpdateStatic + update_time
- *                          (src/sim/SingleGPUSimulator.cu:146-245, src/gpu_utils/gpu_func.cu:
- *                          149-179, src/gpu_utils/runtime.cu:81-92), n cycles per call
+ *                          (src/sim/SingleGPUSimulator.cu:146-245, src/gpu_utils/gpu_func.cu:44-51,
+ *                          67-73, src/gpu_utils/runtime.cu:81-92), n cycles per call
  *      bsim_sim_read_*  <- cudaFetchLIFE / fetchNetworkFromGPU, the GSim.log raster and
  *                          g_v.data trace, gFireCount ("count" of FireInfo)
  *                          (src/neuron/lif/GLIFENeurons.cu:29-40, src/gpu_utils/gpu_net_copy.cu:
@@ -98,7 +98,7 @@ int32_t bsim_net_add_bernoulli_keyed(bsim_network *net, int32_t n, const uint32_
                                      uint32_t first_key);
 
 /* Appends n Poisson sources with the reference's CPU semantics
- * (PoissonNeuron.cpp:37-63): source i first fires at cycle
+ * (PoissonNeuron.cpp:36-63): source i first fires at cycle
  * int(start[i] / dt) + X, then 1 + X cycles after each spike, while the
  * firing cycle is below int((start[i] + duration[i]) / dt); every X is an
  * independent Poisson(rate[i]) draw (rate = the mean number of cycles
@@ -119,6 +119,11 @@ int64_t bsim_net_connect(bsim_network *net, int64_t count, const int32_t *pre,
                          const int8_t *stype, const float *tau);
 
 int32_t bsim_net_num_neurons(const bsim_network *net);
+/* Kind of every neuron (BSIM_KIND_*: 0 LIF_brian, 1 Array_spikes, 2 Bernoulli,
+ * 3 Poisson_spikes) in creation order; returns the count.  The reference
+ * groups neurons by type (Network::buildNetwork, src/net/BuildNetwork.cpp:
+ * 36-94); its -DLOG_DATA traces cover the LIF type. */
+int32_t bsim_net_kinds(const bsim_network *net, int8_t *kind);
 int64_t bsim_net_num_synapses(const bsim_network *net);
 
 /* ---- device simulation ---------------------------------------------------- */
@@ -159,6 +164,15 @@ int64_t bsim_sim_read_raster(bsim_sim *sim, int32_t *counts, int32_t *ids, int64
 
 /* V trace of the recorded neurons: [recorded cycles][n_rec]. */
 int32_t bsim_sim_read_trace(bsim_sim *sim, float *v);
+
+/* State traces of the traced neurons, as the reference's -DLOG_DATA run
+ * writes them per cycle (src/sim/SingleGPUSimulator.cu:150-216): i_E and i_I
+ * after the update (g_ie.data, g_ii.data) and the E / I synaptic input the
+ * update consumed (g_input_e.data, g_input_i.data).  Enable before the first
+ * cycle of a simulation created with traced neurons; read [recorded
+ * cycles][n_rec] per array (NULL arrays skipped). */
+int32_t bsim_sim_enable_state_trace(bsim_sim *sim);
+int32_t bsim_sim_read_state_trace(bsim_sim *sim, float *ie, float *ii, float *in_e, float *in_i);
 
 /* Neuron state (any pointer may be NULL): V, i_E, i_I, refractory counter,
  * spike count. */

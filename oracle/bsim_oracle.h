@@ -8,14 +8,14 @@
  * callers.  The product library never links, loads or calls it.
  *
  * Reference semantics followed (paths relative to the BSim repository):
- *   - LIF constants:       src/neuron/lif/LIFENeuron.cpp:123-167  (reset)
- *   - LIF step:            src/neuron/lif/LIFENeuron.cpp:169-215  (update)
- *   - input split E/I:     src/neuron/lif/LIFENeuron.cpp:107-116  (recv)
- *   - synapse delay/weight src/synapse/static/StaticSynapse.cpp:81-108
- *   - tau choice per type  src/base/CompositeNeuron.h:200-218
+ *   - LIF constants:       src/neuron/lif/LIFENeuron.cpp:68-112  (reset)
+ *   - LIF step:            src/neuron/lif/LIFENeuron.cpp:114-160  (update)
+ *   - input split E/I:     src/neuron/lif/LIFENeuron.cpp:52-61  (recv)
+ *   - synapse delay/weight src/synapse/static/StaticSynapse.cpp:43-70
+ *   - tau choice per type  src/base/CompositeNeuron.h:62-79
  *   - step order:          src/net/Network.cpp:308-330 (neurons in creation
  *                          order, then every synapse in creation order)
- *   - array input source:  src/neuron/array/ArrayNeuron.cpp:115-128
+ *   - array input source:  src/neuron/array/ArrayNeuron.cpp:67-80
  *   - cycle count:         src/sim/SingleThreadSimulator.cpp:23 (round(T/dt))
  *
  * Two entry points per precision:
@@ -43,7 +43,7 @@ enum { BSO_LIF = 0, BSO_ARRAY = 1, BSO_BERNOULLI = 2, BSO_POISSON = 3 };
 /* Number of REAL values per LIF neuron in the parameter block:
  * v_init v_rest v_reset cm tau_m tau_refrac tau_syn_E tau_syn_I v_thresh
  * i_offset, then the LIF_brian composite's own tau_syn_E tau_syn_I
- * (CompositeNeuron.h:182-185), used to pick a synapse's tau. */
+ * (CompositeNeuron.h:43-46), used to pick a synapse's tau. */
 #define BSO_LIF_NPARAM 12
 
 #define BSO_DECLARE_NET(REAL, SUF)                                            \
@@ -94,6 +94,13 @@ typedef struct bso_out_##SUF {                                                \
     int32_t *slot_count;     /* [n] delay slots of the neuron's spikes that  */ \
                              /* the deliveries visited                       */ \
                              /* (src/synapse/static/static.cu:42, :82, :106) */ \
+    /* per-step state traces of the selected neurons (may be NULL): i_E and  */ \
+    /* i_I after the update, and the E / I input the update consumed -- the  */ \
+    /* values LIFENeuron::update pushes into SimInfo (LIFENeuron.cpp:144-148) */ \
+    REAL *rec_ie;            /* [steps * n_rec]                             */ \
+    REAL *rec_ii;                                                             \
+    REAL *rec_in_e;                                                           \
+    REAL *rec_in_i;                                                           \
 } bso_out_##SUF;                                                              \
                                                                               \
 int bso_lif_constants_##SUF(const bso_net_##SUF *net, REAL dt, REAL *out9,   \
@@ -117,7 +124,7 @@ int bso_csr_order_ok(int32_t n, int64_t s, const int32_t *pre,
  * cycle); a source fires at cycle t iff bso_hash(seed, id, t) < p. */
 uint32_t bso_hash(uint64_t seed, uint32_t id, uint32_t t);
 
-/* Poisson sources (src/neuron/poisson/PoissonNeuron.cpp:37-63, drawn from
+/* Poisson sources (src/neuron/poisson/PoissonNeuron.cpp:36-63, drawn from
  * seeded uniforms instead of a clock-seeded engine): X ~ Poisson(rate) is the
  * smallest j with u < floor(P(X <= j) * 2^32), u = bso_hash(seed, id, t).
  * bso_poisson_table returns a malloc'd table {J, thresholds[0 .. J-1]}. */

@@ -4,13 +4,13 @@ Only tests/, ``__graft_entry__.smoke()`` and bench.py's ``cpu_baseline`` leg
 import this module.  It drives ``oracle/liboracle.so`` (built from stsim.c by
 oracle/Makefile), a plain-C restatement of BSim's single-threaded CPU simulator:
 
-* LIF constants   -- src/neuron/lif/LIFENeuron.cpp:123-167
-* LIF step        -- src/neuron/lif/LIFENeuron.cpp:169-215
-* E/I input split -- src/neuron/lif/LIFENeuron.cpp:107-116
-* synapse reset   -- src/synapse/static/StaticSynapse.cpp:81-92
+* LIF constants   -- src/neuron/lif/LIFENeuron.cpp:68-112
+* LIF step        -- src/neuron/lif/LIFENeuron.cpp:114-160
+* E/I input split -- src/neuron/lif/LIFENeuron.cpp:52-61
+* synapse reset   -- src/synapse/static/StaticSynapse.cpp:43-54
 * step order      -- src/net/Network.cpp:308-330
-* array source    -- src/neuron/array/ArrayNeuron.cpp:115-128
-* Poisson source  -- src/neuron/poisson/PoissonNeuron.cpp:37-63 (seeded draws)
+* array source    -- src/neuron/array/ArrayNeuron.cpp:67-80
+* Poisson source  -- src/neuron/poisson/PoissonNeuron.cpp:36-63 (seeded draws)
 
 A network is passed as an object with numpy attributes (``kind``, ``lif``,
 ``arr_off``, ``arr_t``, ``bern_p``, ``seed``, ``pre``, ``post``, ``w``, ``d``,
@@ -87,7 +87,8 @@ def _structs(suf):
                     ("rec_v", P), ("fire_count", P), ("final_v", P), ("final_ie", P),
                     ("final_ii", P), ("final_refrac", P), ("events", C.c_int64),
                     ("time_from", C.c_int32), ("wall_s", C.c_double),
-                    ("events_from", C.c_int64), ("input_sum", P), ("slot_count", P)]
+                    ("events_from", C.c_int64), ("input_sum", P), ("slot_count", P),
+                    ("rec_ie", P), ("rec_ii", P), ("rec_in_e", P), ("rec_in_i", P)]
 
     del real
     _STRUCTS[suf] = (Net, Out)
@@ -148,6 +149,10 @@ class OracleResult:
                                    # the deliveries (static.cu:42, :82, :106)
     wall_s: float = 0.0        # seconds spent in cycles >= time_from
     events_from: int = 0       # deliveries in cycles >= time_from
+    rec_ie: np.ndarray = None    # [steps, n_rec] i_E after each cycle's update
+    rec_ii: np.ndarray = None    # [steps, n_rec] i_I after each cycle's update
+    rec_in_e: np.ndarray = None  # [steps, n_rec] E input each cycle's update consumed
+    rec_in_i: np.ndarray = None  # [steps, n_rec] I input each cycle's update consumed
 
     def raster(self):
         """List of per-cycle ascending fired-id arrays."""
@@ -170,6 +175,10 @@ def run(net, dt, steps, rec_ids=(), dtype=np.float32, mode="exact", tile=4096, t
     res = dict(raster_count=np.zeros(steps, np.int32),
                raster_ids=np.zeros(raster_cap, np.int32),
                rec_v=np.zeros((steps, len(rec)), a.dtype),
+               rec_ie=np.zeros((steps, len(rec)), a.dtype),
+               rec_ii=np.zeros((steps, len(rec)), a.dtype),
+               rec_in_e=np.zeros((steps, len(rec)), a.dtype),
+               rec_in_i=np.zeros((steps, len(rec)), a.dtype),
                fire_count=np.zeros(n, np.int32),
                final_v=np.zeros(n, a.dtype), final_ie=np.zeros(n, a.dtype),
                final_ii=np.zeros(n, a.dtype), final_refrac=np.zeros(n, np.int32),
@@ -180,7 +189,9 @@ def run(net, dt, steps, rec_ids=(), dtype=np.float32, mode="exact", tile=4096, t
                   _ptr(res["rec_v"]) if len(rec) else None, _ptr(res["fire_count"]),
                   _ptr(res["final_v"]), _ptr(res["final_ie"]), _ptr(res["final_ii"]),
                   _ptr(res["final_refrac"]), 0, int(time_from), 0.0, 0,
-                  _ptr(res["input_sum"]), _ptr(res["slot_count"]))
+                  _ptr(res["input_sum"]), _ptr(res["slot_count"]),
+                  *[_ptr(res[k]) if len(rec) else None
+                    for k in ("rec_ie", "rec_ii", "rec_in_e", "rec_in_i")])
         dt_c = float(np.asarray(dt, a.dtype))
         if mode == "exact":
             rc = getattr(L, f"bso_run_{suf}")(C.byref(a.c), dt_c, steps, C.byref(out))

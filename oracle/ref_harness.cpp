@@ -8,7 +8,7 @@
  * repeats SingleThreadSimulator::run's loop (src/sim/SingleThreadSimulator.cpp:
  * 74-96): Network::reset once, then Network::update per cycle, reading the
  * per-cycle SimInfo that LIFENeuron::update fills (vm, i_syn_E, i_syn_I, i_E,
- * i_I per LIF neuron, LIFENeuron.cpp:199-210) and the fired flags.
+ * i_I per LIF neuron, LIFENeuron.cpp:144-155) and the fired flags.
  *
  * Usage: ref_stsim <input.bsnt> <output.bsout>
  *        ref_stsim --time <input.bsnt>   (times STSim::run, LOG_DATA off)
@@ -30,8 +30,8 @@
 #include "base/CompositeNeuron.h"
 #include "third_party/json/json.h"
 
-typedef CompositeNeuron<LIFENeuron, StaticSynapse> LIF_brian;       // include/Neuron.h:47
-typedef CompositeNeuron<ArrayNeuron, StaticSynapse> Array_spikes;   // include/Neuron.h:43
+typedef CompositeNeuron<LIFENeuron, StaticSynapse> LIF_brian;       // include/Neuron.h:31
+typedef CompositeNeuron<ArrayNeuron, StaticSynapse> Array_spikes;   // include/Neuron.h:27
 
 static FILE *xopen(const char *p, const char *m)
 {
@@ -202,7 +202,10 @@ int main(int argc, char **argv)
     net.reset(info);
 
     std::vector<int32_t> rcount(in.steps), rids;
-    std::vector<real> recv((size_t)in.steps * in.n_rec, 0);
+    /* per cycle and recorded neuron, the five values LIFENeuron::update pushes
+     * into SimInfo (LIFENeuron.cpp:144-148): V, the E and I input it consumed,
+     * i_E and i_I after the update */
+    std::vector<real> recv((size_t)in.steps * in.n_rec * 5, 0);
     for (int32_t t = 0; t < in.steps; t++) {
         info.currCycle = t;
         info.fired.clear();
@@ -217,7 +220,9 @@ int main(int argc, char **argv)
         rcount[t] = c;
         for (int32_t r = 0; r < in.n_rec; r++) {
             int32_t col = lif_col[in.rec[r]];
-            recv[(size_t)t * in.n_rec + r] = col >= 0 ? info.input[(size_t)col * 5] : 0;
+            for (int q = 0; q < 5; q++)
+                recv[((size_t)q * in.steps + t) * in.n_rec + r] =
+                    col >= 0 ? info.input[(size_t)col * 5 + q] : 0;
         }
     }
 
@@ -227,7 +232,7 @@ int main(int argc, char **argv)
     wr(f, &len, 1);
     wr(f, rids.data(), rids.size());
     wr(f, recv.data(), recv.size());
-    // LIF constants (getData: LIFENeuron.cpp:226-245) as the reference holds them
+    // LIF constants (getData: LIFENeuron.cpp:171-190) as the reference holds them
     for (int32_t i = 0; i < in.n; i++) {
         real c9[9] = {0, 0, 0, 0, 0, 0, 0, 0, 0};
         int32_t rt = 0;

@@ -4,7 +4,7 @@
 harness; ``run_ref`` runs the harness built by oracle/build_ref.sh and parses
 its output.  Bernoulli sources are handed to the reference as Array_spikes
 neurons whose cycle lists are the cycles the source hash selects (both fire at
-most once per cycle, src/neuron/array/ArrayNeuron.cpp:115-128).
+most once per cycle, src/neuron/array/ArrayNeuron.cpp:67-80).
 """
 from __future__ import annotations
 
@@ -91,7 +91,9 @@ def read_output(path, n, s, steps, n_rec, dtype=np.float32):
     res["raster_count"] = take(np.int32, steps)
     ln = int(take(np.int64, 1)[0])
     res["raster_ids"] = take(np.int32, ln)
-    res["rec_v"] = take(dtype, steps * n_rec).reshape(steps, n_rec)
+    # V, the E / I input the update consumed, i_E, i_I (LIFENeuron.cpp:144-148)
+    for key in ("rec_v", "rec_in_e", "rec_in_i", "rec_ie", "rec_ii"):
+        res[key] = take(dtype, steps * n_rec).reshape(steps, n_rec)
     rec = np.dtype([("c", dtype, 9), ("rt", np.int32)])
     cr = take(rec, n)
     res["consts"] = cr["c"]

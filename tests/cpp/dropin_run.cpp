@@ -10,15 +10,19 @@
  *   int32 nproj; per projection: int32 src, dst, float w[n_src * n_dst],
  *   float d[n_src * n_dst]; float dt, T; int32 nmon, int32 gid[nmon].
  *
- * usage: dropin_run <input> <output dir> [plain]
+ * usage: dropin_run <input> <output dir> [plain | logdata]
  *   default: run(T, log) with the FireInfo keys "count", "Y" and "X", written
  *            to GFire.count, input_Y.txt (hex floats) and input_X.txt;
- *   plain:   run(T), which writes GFire.count itself.
+ *   plain:   run(T), which writes GFire.count itself;
+ *   logdata: setLogData(true) and run(T): the reference's -DLOG_DATA files
+ *            (GSim.log, g_v.data, g_ie.data, g_ii.data, g_input_e.data,
+ *            g_input_i.data for every LIF neuron).
  */
 #include <stdio.h>
 #include <stdlib.h>
 #include <unistd.h>
 
+#include <string>
 #include <vector>
 
 #include "BSim.h"
@@ -35,7 +39,7 @@ static void rd(FILE *f, T *p, size_t n)
 int main(int argc, char **argv)
 {
     if (argc != 3 && argc != 4) {
-        fprintf(stderr, "usage: %s <input> <output dir> [plain]\n", argv[0]);
+        fprintf(stderr, "usage: %s <input> <output dir> [plain | logdata]\n", argv[0]);
         return 2;
     }
     FILE *f = fopen(argv[1], "rb");
@@ -110,7 +114,8 @@ int main(int argc, char **argv)
     SGSim sg(&c, dt);
     sg.setLog(true);
     sg.monitor(mon);
-    if (argc > 3) {   /* plain */
+    if (argc > 3 && std::string(argv[3]) == "logdata") sg.setLogData(true);
+    if (argc > 3) {   /* plain / logdata */
         try {
             if (sg.run(T) != 0) return 1;
         } catch (const bsim_error &e) {

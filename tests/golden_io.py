@@ -31,6 +31,8 @@ def load(name):
     z = np.load(os.path.join(GOLDEN, name + ".npz"), allow_pickle=False)
     meta = json.loads(bytes(z["meta"]).decode())
     kw = dict(meta["kwargs"])
+    if "w" in kw:
+        kw["w"] = tuple(kw["w"])
     if "wdtype" in kw:
         kw["wdtype"] = np.dtype(kw["wdtype"]).type
     if "i_offset" in kw:
@@ -40,6 +42,9 @@ def load(name):
     data = {k: z[k] for k in z.files if k != "meta"}
     fl = np.float32 if dtype == np.float32 else np.float64
     data["rec_v"] = data["rec_v_bits"].view(fl)
+    for k in ("rec_ie", "rec_ii", "rec_in_e", "rec_in_i"):
+        if k + "_bits" in data:
+            data[k] = data[k + "_bits"].view(fl)
     data["consts"] = data["consts_bits"].view(fl)
     if "syn_w_bits" in data:
         data["syn_w"] = data["syn_w_bits"].view(fl)

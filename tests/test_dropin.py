@@ -29,6 +29,27 @@ def test_header_compiles_standalone(tmp_path):
     assert r.returncode == 0, r.stderr
 
 
+def test_stsim_is_a_compile_error(tmp_path):
+    """STSim (the reference's CPU simulator) is not provided: a program that
+    constructs one fails to compile with a message naming SGSim, while a
+    program that only names the type still compiles."""
+    bad = tmp_path / "st.cpp"
+    bad.write_text("""#include "BSim.h"
+int main() { Network net; STSim st(&net, 1e-4f); return 0; }
+""")
+    r = subprocess.run(["g++", "-std=c++11", "-fsyntax-only", "-I", os.path.join(ROOT, "include"),
+                        str(bad)], capture_output=True, text=True)
+    assert r.returncode != 0
+    assert "use SGSim" in r.stderr
+    ok = tmp_path / "ok.cpp"
+    ok.write_text("""#include "BSim.h"
+int main() { Network net; SGSim sg(&net, 1e-4f); STSim *unused = 0; (void)unused; return 0; }
+""")
+    r = subprocess.run(["g++", "-std=c++11", "-Wall", "-Werror", "-fsyntax-only", "-I",
+                        os.path.join(ROOT, "include"), str(ok)], capture_output=True, text=True)
+    assert r.returncode == 0, r.stderr
+
+
 def test_poisson_population_compiles(tmp_path):
     """Poisson_spikes populations (PoissonNeuron(rate, refract, startTime,
     duration), src/neuron/poisson/PoissonNeuron.cpp:13) build through the
@@ -112,6 +133,29 @@ def test_dropin_program_matches_reference_c0(tmp_path):
     assert np.array_equal(y.view(np.uint32), o.input_sum.view(np.uint32))
     np.testing.assert_array_equal(x, o.slot_count)
     assert x.sum() > 0 and np.count_nonzero(y) > 0
+    # setLogData(true): the reference's -DLOG_DATA files for every LIF neuron
+    # (src/sim/SingleGPUSimulator.cu:150-232), against the reference's own
+    # values for the traced neurons and the oracle's for all of them
+    logd = tmp_path / "logdata"
+    logd.mkdir()
+    r = subprocess.run([BIN, str(inp), str(logd), "logdata"], capture_output=True, text=True,
+                       timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = (logd / "GSim.log").read_text().split("\n")[:steps]
+    ids2 = np.array([int(x) for l in lines for x in l.split()], np.int32)
+    np.testing.assert_array_equal(ids2, gold["raster_ids"])
+    allrec = list(range(2 * n))
+    o2 = oracle.run(spec, netspec.DT, steps, rec_ids=allrec)
+    for fname, gkey, okey in (("g_v.data", "rec_v", "rec_v"), ("g_ie.data", "rec_ie", "rec_ie"),
+                              ("g_ii.data", "rec_ii", "rec_ii"),
+                              ("g_input_e.data", "rec_in_e", "rec_in_e"),
+                              ("g_input_i.data", "rec_in_i", "rec_in_i")):
+        a = np.loadtxt(logd / fname, ndmin=2)
+        assert a.shape == (steps, 2 * n), (fname, a.shape)
+        np.testing.assert_allclose(a, getattr(o2, okey).astype(np.float64), rtol=0, atol=5.1e-11,
+                                   err_msg=fname)
+        np.testing.assert_allclose(a[:, rec], gold[gkey].astype(np.float64), rtol=0,
+                                   atol=5.1e-11, err_msg=fname)
     # run(T) writes GFire.count by itself
     plain = tmp_path / "plain"
     plain.mkdir()
@@ -176,3 +220,38 @@ def test_dropin_poisson_drive_matches_oracle(tmp_path):
     fc = np.array((tmp_path / "GFire.count").read_text().split(), np.int32)
     np.testing.assert_array_equal(fc, o.fire_count)
     assert fc[n0 + n1:].sum() > 0 and fc[:n0].sum() > 0
+
+
+STD_BIN = os.path.join(ROOT, "tests", "cpp", "standard_run")
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("name", ["standard_test_f32", "standard_test_hi_f32"])
+def test_standard_topology_program_matches_reference(tmp_path, name):
+    """tests/cpp/standard_run.cpp builds the reference's benchmark topology
+    (test/gpu/standard_test.cpp:55-138: g[i-1] -> g[i] connected before
+    g[1] -> g[i], inhibitory side populations) through include/BSim.h;
+    on the GPU its GSim.log raster, GFire.count and g_v.data equal the
+    golden vectors recorded from the reference's own CPU simulator."""
+    if not os.path.exists(STD_BIN):
+        pytest.fail("tests/cpp/standard_run is not built (run __graft_entry__.build())")
+    meta, spec, dtype, gold = golden_io.load(name)
+    kw = meta["kwargs"]
+    steps = meta["steps"]
+    T = np.float32(steps * float(np.float32(netspec.DT)))
+    rec = [str(i) for i in meta["rec_ids"]]
+    args = [STD_BIN, str(kw["depth"]), str(kw["n"]), *[repr(float(x)) for x in kw["w"]],
+            str(kw["who"]), repr(float(T)), str(tmp_path), *rec]
+    print("[gpu] standard_run", " ".join(args[1:]), flush=True)
+    r = subprocess.run(args, capture_output=True, text=True, timeout=300)
+    assert r.returncode == 0, r.stdout + r.stderr
+    lines = (tmp_path / "GSim.log").read_text().split("\n")[:steps]
+    counts = np.array([len(l.split()) for l in lines])
+    ids = np.array([int(x) for l in lines for x in l.split()], np.int32)
+    np.testing.assert_array_equal(counts, gold["raster_count"])
+    np.testing.assert_array_equal(ids, gold["raster_ids"])
+    fc = np.array((tmp_path / "GFire.count").read_text().split(), np.int32)
+    np.testing.assert_array_equal(fc, gold["fire_count"])
+    gv = np.loadtxt(tmp_path / "g_v.data", ndmin=2)
+    np.testing.assert_allclose(gv, gold["rec_v"].astype(np.float64), rtol=0, atol=5.1e-11)
+    assert len(ids) > 1000

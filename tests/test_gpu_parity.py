@@ -26,17 +26,28 @@ def say(*a):
 def run_gpu(spec, steps, rec_ids, tile=0):
     net = api.Network.from_spec(spec)
     sim = api.Simulation(net, DT, tile=tile, rec_steps=steps, rec_ids=rec_ids)
+    if len(rec_ids):
+        sim.enable_state_trace()
     sim.step(steps)
     sim.finish()
     sim.sync()
     counts, ids, _ = sim.raster()
     out = dict(raster_count=counts, raster_ids=ids, rec_v=sim.trace(), events=sim.events(),
                info=sim.info(), **sim.state())
+    if len(rec_ids):
+        out.update({"rec_" + k: v for k, v in sim.state_trace().items()})
     sim.close()
     return out
 
 
+STATE_TRACES = ("rec_ie", "rec_ii", "rec_in_e", "rec_in_i")
+
+
 def assert_same(g, o, what):
+    for k in STATE_TRACES:
+        if k in g:
+            assert np.array_equal(g[k].view(np.uint32), getattr(o, k).view(np.uint32)), \
+                f"{what}: {k}"
     np.testing.assert_array_equal(g["raster_count"], o.raster_count, err_msg=f"{what}: counts")
     np.testing.assert_array_equal(g["raster_ids"], o.raster_ids, err_msg=f"{what}: ids")
     assert np.array_equal(g["rec_v"].view(np.uint32), o.rec_v.view(np.uint32)), f"{what}: V"
@@ -58,16 +69,14 @@ def test_golden_cases(name, tile):
     """GPU == reference CPU build, bit for bit."""
     meta, spec, dtype, g = golden_io.load(name)
     say(f"[gpu] golden {name} tile={tile} n={spec.n} s={spec.s} steps={meta['steps']}")
-    _, ds = oracle.synapse_reset(spec, DT)
-    if not oracle.csr_order_ok(spec, ds):
-        # created out of (pre, delay) order: the GPU step refuses it up front
-        with pytest.raises(api.BsimError, match=r"ascending \(pre, delay\) order"):
-            run_gpu(spec, meta["steps"], meta["rec_ids"], tile)
-        return
+    # shuffled creation orders included: each target's inputs are summed in
+    # its own creation order
     r = run_gpu(spec, meta["steps"], meta["rec_ids"], tile)
     np.testing.assert_array_equal(r["raster_count"], g["raster_count"])
     np.testing.assert_array_equal(r["raster_ids"], g["raster_ids"])
     assert np.array_equal(r["rec_v"].view(np.uint32), g["rec_v_bits"])
+    for k in STATE_TRACES:   # i_E / i_I after the update, the input it consumed
+        assert np.array_equal(r[k].view(np.uint32), g[k + "_bits"]), k
     np.testing.assert_array_equal(r["fire_count"], g["fire_count"])
     lif = spec.kind == netspec.LIF
     assert np.array_equal(r["v"].view(np.uint32)[lif], g["consts_bits"][lif][:, 8])

@@ -72,6 +72,61 @@ def test_layout_is_pre_major_within_groups():
     assert ngroups >= 1
 
 
+def _segments(pre, post, ds):
+    """Per synapse (creation order): the index of the run of ascending (pre,
+    delay) order it belongs to among its target's inputs."""
+    last, seg = {}, {}
+    out = np.zeros(len(pre), np.int64)
+    for k in range(len(pre)):
+        j, key = int(post[k]), (int(pre[k]), int(ds[k]))
+        if j in last and key < last[j]:
+            seg[j] = seg.get(j, 0) + 1
+        last[j] = key
+        out[k] = seg.get(j, 0)
+    return out
+
+
+@pytest.mark.parametrize("which", ["standard_test", "shuffled"])
+def test_layout_orders_each_target_by_creation_runs(which):
+    """Networks whose targets are fed by a later population first and an
+    earlier one second (test/gpu/standard_test.cpp:116-121), or in any
+    creation order: the group block is sorted by (run of ascending (pre,
+    delay) order in the target's creation sequence, pre, delay, tile, part,
+    creation), which makes each target's stream order its creation order."""
+    spec = (netspec.standard_test(depth=7, n=60, who=3) if which == "standard_test" else
+            netspec.random_net(3, shuffle=True, n_lif=60, p_conn=0.25))
+    net = api.Network.from_spec(spec)
+    tile = 192
+    L = net.layout(DT, tile)
+    w_ref, ds = oracle.synapse_reset(spec, DT)
+    seg = _segments(spec.pre, spec.post, ds)
+    if which == "standard_test":
+        assert seg.max() == 1
+    else:
+        assert seg.max() >= 3
+    ts = L["tile_syn"]
+    k = 0
+    while k < L["ntiles"]:
+        k1 = k
+        while k1 < L["ntiles"] and ts[k1] == ts[k]:
+            k1 += 1
+        a, b = ts[k], (ts[k1] if k1 < L["ntiles"] else L["s"])
+        sel = np.nonzero((spec.post >= k * tile) & (spec.post < k1 * tile))[0]
+        tl = spec.post[sel] // tile
+        part = (spec.post[sel] - tl * tile) // (tile // 3)
+        # the ranges of one segment (ascending pres) come before those of the
+        # next
+        order = sel[np.lexsort((sel, part, tl, ds[sel], spec.pre[sel], seg[sel]))]
+        assert b - a == len(order)
+        np.testing.assert_array_equal(L["syn_tgt"][a:b], spec.post[order] % tile)
+        assert np.array_equal(L["syn_w"][a:b].view(np.uint32), w_ref[order].view(np.uint32))
+        # each target's inputs, in stream order, are its creation order
+        for j in np.unique(spec.post[sel]):
+            mine = order[spec.post[order] == j]
+            assert np.all(np.diff(mine) > 0)
+        k = k1
+
+
 def test_layout_single_tile_matches_reference_csr():
     """test/lif_unit1_test.py:105-106: p_dst = [10..19] x 10 + [20..29] x 10."""
     net = api.Network.from_spec(netspec.lif_unit(1))
@@ -114,11 +169,21 @@ def test_nonintegral_delay_rescales_weight_like_reference():
     assert L["dmax"] == ds[0]
 
 
-def test_rejects_creation_order_against_pre_order():
-    spec = netspec.random_net(1, shuffle=True, n_lif=60, p_conn=0.3)
-    net = api.Network.from_spec(spec)
-    with pytest.raises(api.BsimError, match="ascending \\(pre, delay\\) order"):
+def test_rejects_more_creation_runs_than_the_layout_keeps():
+    """A target whose 40 inputs were created in descending pre order has 40
+    runs of ascending (pre, delay) order; the layout keeps 32 per target."""
+    b = netspec.Builder()
+    b.population(np.tile(netspec.lif_row(0, 0, 0, 0.02, 0.02, 0, 5e-3, 5e-3, 1.0, 0.5), (41, 1)))
+    pre = np.arange(39, -1, -1)
+    b.connect(pre, np.full(40, 40), np.full(40, 0.1, np.float32), np.full(40, DT, np.float32))
+    net = api.Network.from_spec(b.build())
+    with pytest.raises(api.BsimError, match="at most 32 such runs per target"):
         net.layout(DT, 192)
+    # 32 runs are accepted
+    b2 = netspec.Builder()
+    b2.population(np.tile(netspec.lif_row(0, 0, 0, 0.02, 0.02, 0, 5e-3, 5e-3, 1.0, 0.5), (41, 1)))
+    b2.connect(pre[8:], np.full(32, 40), np.full(32, 0.1, np.float32), np.full(32, DT, np.float32))
+    api.Network.from_spec(b2.build()).layout(DT, 192)
 
 
 def test_bad_arguments():
@@ -131,3 +196,31 @@ def test_bad_arguments():
         n2 = api.Network.from_spec(netspec.lif_unit(1))
         n2.connect([0], [999], [0.1], [1e-4])
         n2.layout(DT, 192)
+
+
+def test_run_cycles_rounds_halves_away_from_zero():
+    """Simulation.run's cycle count is C's round() of the float quotient
+    (SingleGPUSimulator.cu:34), checked against libm's roundf itself on
+    quotients at and around x.5 step boundaries."""
+    import ctypes
+    libm = ctypes.CDLL("libm.so.6")
+    libm.roundf.restype = ctypes.c_float
+    libm.roundf.argtypes = [ctypes.c_float]
+    dt = np.float32(1e-4)
+    rng = np.random.default_rng(5)
+    ks = np.concatenate([np.arange(0, 2000), rng.integers(0, 200000, 4000)])
+    Ts = []
+    for k in ks:
+        q = np.float32(k + 0.5)
+        t = q * dt
+        Ts += [t, np.nextafter(t, np.float32(0)), np.nextafter(t, np.float32(1))]
+    Ts += [np.float32(0.45), np.float32(0.1), np.float32(1.0), np.float32(0.0)]
+    mism = 0
+    for t in Ts:
+        q = np.float32(np.float32(t) / dt)
+        want = int(libm.roundf(float(q)))
+        mism += api.run_cycles(t, dt) != want
+    assert mism == 0
+    # the case numpy's round-half-to-even gets wrong: 4.5 steps -> 5 cycles
+    assert api.run_cycles(np.float32(4.5) * dt / np.float32(1.0), dt) == \
+        int(libm.roundf(float(np.float32(np.float32(4.5) * dt) / dt)))

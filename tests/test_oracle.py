@@ -53,6 +53,9 @@ def test_oracle_matches_reference_build(name):
     np.testing.assert_array_equal(r.raster_count, g["raster_count"])
     np.testing.assert_array_equal(r.raster_ids, g["raster_ids"])
     assert np.array_equal(r.rec_v.view(g["rec_v_bits"].dtype), g["rec_v_bits"])
+    for k in ("rec_ie", "rec_ii", "rec_in_e", "rec_in_i"):
+        if k + "_bits" in g:   # i_E / i_I after the update, the input it consumed
+            assert np.array_equal(getattr(r, k).view(g[k + "_bits"].dtype), g[k + "_bits"]), k
     np.testing.assert_array_equal(r.fire_count, g["fire_count"])
     c9, rt = oracle.lif_constants(net, meta["dt"], dtype)
     lif = net.kind == netspec.LIF
@@ -115,8 +118,8 @@ def test_constants_closed_form():
 def test_delay_timing_closed_form(d):
     """A spike sent at cycle s through a delay of d steps is added to the
     target's input in phase (ii) of cycle s + d, enters i_E at s + d + 1 and
-    first moves V at s + d + 2 (src/synapse/static/StaticSynapse.cpp:94-108,
-    src/neuron/lif/LIFENeuron.cpp:169-215)."""
+    first moves V at s + d + 2 (src/synapse/static/StaticSynapse.cpp:56-70,
+    src/neuron/lif/LIFENeuron.cpp:114-160)."""
     s = 5
     b = netspec.Builder()
     b.population(np.zeros((1, netspec.LIF_NPARAM)), kind=netspec.ARRAY, spikes=[[s]])
@@ -132,7 +135,7 @@ def test_delay_timing_closed_form(d):
 
 def test_refractory_discards_input():
     """A refractory neuron only counts down and its input is dropped
-    (LIFENeuron.cpp:173-174, 207-208): a target held refractory while the
+    (LIFENeuron.cpp:118-119, 152-153): a target held refractory while the
     spike arrives never moves."""
     b = netspec.Builder()
     b.population(np.zeros((1, netspec.LIF_NPARAM)), kind=netspec.ARRAY, spikes=[[0]])
@@ -260,3 +263,55 @@ def test_input_sum_closed_form():
     assert np.all(o.input_sum[:8] == 0)
     oc = oracle.run(spec, DT, steps, mode="csr", tile=64, threads=2)
     assert np.array_equal(oc.input_sum.view(np.uint32), o.input_sum.view(np.uint32))
+
+
+@pytest.mark.parametrize("name", ["nonint1_f32", "nonint2_f32"])
+def test_nonint_golden_pins_rescale_and_tau_choice(name):
+    """The non-integral-delay goldens exercise every branch of
+    StaticSynapse::reset's rescale (src/synapse/static/StaticSynapse.cpp:43-54)
+    and of CompositeNeuron::createSynapse's tau choice (src/base/
+    CompositeNeuron.h:62-79): rescaled weights with an explicit tau, with the
+    destination's excitatory and with its inhibitory composite tau; and the
+    reference's weights differ from what a plausible mistake gives -- the
+    other composite tau, the neuron's own tau_syn instead of the composite's,
+    or no rescale -- so each of those fails the golden."""
+    meta, net, dtype, g = golden_io.load(name)
+    gw = g["syn_w"]
+    ds = g["syn_ds"]
+    f = np.float32
+    w0 = net.w.astype(f)
+    d = net.d.astype(f)
+    dt = f(meta["dt"])
+    resid = d - dt * ds.astype(f)
+    moved = np.abs(resid) > f(1e-10)
+    tau_e = net.lif[net.post, 10].astype(f)
+    tau_i = net.lif[net.post, 11].astype(f)
+    own_e = net.lif[net.post, 6].astype(f)
+    own_i = net.lif[net.post, 7].astype(f)
+    expl = np.abs(net.tau.astype(f)) > f(1e-10)
+    inh = net.stype == 1
+    rescaled = moved & (gw.view(np.uint32) != w0.view(np.uint32))
+    for sel in (expl, ~expl & ~inh, ~expl & inh):
+        assert np.count_nonzero(rescaled & sel) >= 20
+
+    import ctypes
+    libm = ctypes.CDLL("libm.so.6")
+    libm.expf.restype = ctypes.c_float
+    libm.expf.argtypes = [ctypes.c_float]
+
+    def apply(tau):
+        out = w0.copy()
+        m = moved & (np.abs(tau) > f(1e-10))
+        x = (-(resid[m]) / tau[m]).astype(f)
+        c = np.array([libm.expf(float(v)) for v in x], f)
+        out[m] = (w0[m] * c).astype(f)
+        return out
+
+    def tau_with(e, i):
+        return np.where(expl, net.tau.astype(f), np.where(inh, i, e)).astype(f)
+
+    right = apply(tau_with(tau_e, tau_i))
+    assert np.array_equal(right.view(np.uint32), gw.view(np.uint32))
+    for wrong in (apply(tau_with(tau_i, tau_e)), apply(tau_with(own_e, own_i)), w0):
+        diff = np.abs(wrong.view(np.int32).astype(np.int64) - gw.view(np.int32).astype(np.int64))
+        assert np.count_nonzero(diff > 1) >= 20

@@ -198,14 +198,15 @@ def _rel(a, b):
 
 
 @pytest.mark.gpu
-def test_gpu_forward_matches_oracle():
-    m = _model()
+@pytest.mark.parametrize("batch", [1000, 4096])   # 1000: not a multiple of 64
+def test_gpu_forward_matches_oracle(batch):
+    m = _model(max_batch=max(batch, 1024))
     flat = m.get_params()
     rng = np.random.default_rng(8)
-    x = rng.normal(size=(1000, 8)).astype(np.float32)   # not a multiple of 64
+    x = rng.normal(size=(batch, 8)).astype(np.float32)
     y = m.predict(x)
     ref, _ = mo.forward(flat, m.dims, x, emulate=True)
-    assert y.shape == (1000, 200)
+    assert y.shape == (batch, 200)
     assert _rel(y, ref[:, :200]) < 2e-3
     # per-sample: a transposed or shifted output tile would fail here
     np.testing.assert_allclose(y, ref[:, :200], rtol=2e-2, atol=2e-2 * np.abs(ref).max())
@@ -213,12 +214,13 @@ def test_gpu_forward_matches_oracle():
 
 
 @pytest.mark.gpu
-def test_gpu_gradients_match_oracle():
-    m = _model()
+@pytest.mark.parametrize("batch", [512, 4096])    # 4096: the training batch
+def test_gpu_gradients_match_oracle(batch):
+    m = _model(max_batch=max(batch, 1024))
     flat = m.get_params()
     rng = np.random.default_rng(9)
-    x = rng.normal(size=(512, 8)).astype(np.float32)
-    y = rng.normal(size=(512, 200)).astype(np.float32)
+    x = rng.normal(size=(batch, 8)).astype(np.float32)
+    y = rng.normal(size=(batch, 200)).astype(np.float32)
     g, loss = m.gradients(x, y)
     rl, rg = mo.loss_and_grad(flat, m.dims, x, y, emulate=True)
     assert abs(loss - rl) <= 1e-4 * rl
@@ -234,15 +236,16 @@ def test_gpu_gradients_match_oracle():
 
 
 @pytest.mark.gpu
-def test_gpu_adam_step_matches_oracle():
-    m = _model()
+@pytest.mark.parametrize("batch", [512, 4096])
+def test_gpu_adam_step_matches_oracle(batch):
+    m = _model(max_batch=max(batch, 1024))
     p0 = m.get_params()
     rng = np.random.default_rng(10)
-    x = rng.normal(size=(512, 8)).astype(np.float32)
-    y = rng.normal(size=(512, 200)).astype(np.float32)
+    x = rng.normal(size=(batch, 8)).astype(np.float32)
+    y = rng.normal(size=(batch, 200)).astype(np.float32)
     g, loss = m.gradients(x, y)
     m.set_data(x, y)
-    losses = m.train(np.arange(512, dtype=np.int32)[None, :], 512, 1e-3)
+    losses = m.train(np.arange(batch, dtype=np.int32)[None, :], batch, 1e-3)
     assert losses[0] == loss
     p1 = m.get_params()
     want, _, _ = mo.adam(p0, np.zeros_like(p0), np.zeros_like(p0), g, 1, 1e-3)

@@ -108,3 +108,34 @@ def test_sweep_network_alone_matches_oracle():
         o = oracle.run(spec, DT, steps)
         np.testing.assert_array_equal(np.array([len(x) for x in solo]), o.raster_count)
         np.testing.assert_array_equal(np.concatenate(solo), o.raster_ids)
+
+
+@pytest.mark.gpu
+@pytest.mark.timeout(400)
+def test_full_size_sweep_batch_equals_solo():
+    """Networks drawn from the sweep's full distribution (2-8 populations of
+    1,000-50,000 neurons, fan-in 100-1,000, delays up to 6..20 steps), 2,000
+    cycles (0.2 s) in one batched simulation; a sample of them run alone is
+    bit for bit the same, and one of those equals the oracle."""
+    steps = 2000
+    pts = sweep.sample(40, seed=17)
+    nsyn = sum(p.fan_in * sum(p.sizes[1:]) + p.drive_fan_in * p.sizes[0] for p in pts)
+    print(f"[gpu] full-size sweep batch: {len(pts)} networks, {sum(p.n for p in pts)} neurons, "
+          f"~{nsyn:.3g} synapses", flush=True)
+    rasters, secs, info = sweep.run_batch(pts, steps)
+    print(f"[gpu] batch stepped in {secs:.2f}s, tile {info['tile']}", flush=True)
+    rng = np.random.default_rng(3)
+    pick = sorted(rng.choice(len(pts), 5, replace=False).tolist())
+    spikes = 0
+    for k in pick:
+        solo, _ = sweep.run_solo(pts[k], steps)
+        for c in range(steps):
+            np.testing.assert_array_equal(rasters[k][c], solo[c], err_msg=f"network {k} cycle {c}")
+        spikes += sum(len(x) for x in solo)
+    assert spikes > 1000
+    k = min(pick, key=lambda i: pts[i].n)
+    net = api.Network()
+    net.add_chain(**pts[k].kwargs())
+    o = oracle.run(net.export_spec(), DT, steps, mode="csr", tile=2048, threads=8)
+    np.testing.assert_array_equal(np.array([len(x) for x in rasters[k]]), o.raster_count)
+    np.testing.assert_array_equal(np.concatenate(rasters[k]), o.raster_ids)
