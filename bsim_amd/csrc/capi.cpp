@@ -371,10 +371,9 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
         d.field = dev_upload(sim, vec, e);               \
         if (e != hipSuccess) goto hip_fail;              \
     } while (0)
-    UP(kind, L.kind);
-    UP(Cm, L.Cm); UP(CE, L.CE); UP(CI, L.CI); UP(C_E, L.C_E); UP(C_I, L.C_I);
-    UP(v_tmp, L.v_tmp); UP(v_thresh, L.v_thresh); UP(v_reset, L.v_reset);
-    UP(refrac_time, L.refrac_time);
+    UP(ncls, L.ncls);
+    UP(lifc, L.lifc);
+    UP(v_tmp, L.v_tmp);
     UP(arr_end, L.arr_end);
     UP(arr_t, L.arr_t);
     UP(bern_p, L.bern_p);

@@ -78,6 +78,17 @@ typedef struct __attribute__((aligned(8))) bsim_syn {
 #define BSIM_MAX_RANGE (1 << 26)   /* pres per source range: row index fits 32 bits */
 #define BSIM_CHUNK 4096             /* pres per source chunk (one row-list workgroup) */
 
+/* LIF_brian constants shared by the neurons of one parameter set (usually a
+ * population): the reference keeps them per neuron (GLIFENeurons.h:7-24);
+ * here a neuron carries only its set and its own v_tmp (which holds its
+ * i_offset). */
+typedef struct __attribute__((aligned(16))) bsim_lifc {
+    float Cm, CE, CI, C_E, C_I, v_thresh, v_reset;
+    int32_t refrac_time;
+} bsim_lifc;
+
+#define BSIM_CLS_SET_MASK 0x3FFFFFFFu   /* ncls: kind << 30 | parameter set */
+
 /* Kernel arguments (passed by value). */
 typedef struct bsim_dev {
     int32_t n;           /* neurons */
@@ -86,10 +97,11 @@ typedef struct bsim_dev {
     int32_t dmax;        /* D, 1..BSIM_MAX_DELAY */
     int32_t nwords;      /* (n + 63) / 64 */
     int32_t pad0;
-    /* neuron constants (per neuron) */
-    const int8_t *kind;
-    const float *Cm, *CE, *CI, *C_E, *C_I, *v_tmp, *v_thresh, *v_reset;
-    const int32_t *refrac_time;
+    /* neuron constants: kind and LIF parameter set per neuron, the sets'
+     * shared constants, v_tmp per neuron */
+    const uint32_t *ncls;       /* kind << 30 | index into lifc (LIF neurons) */
+    const bsim_lifc *lifc;
+    const float *v_tmp;
     const int32_t *arr_end;     /* array sources: end index into arr_t */
     const int32_t *arr_t;
     const uint32_t *bern_p;

@@ -14,6 +14,8 @@
 #include <algorithm>
 #include <map>
 #include <memory>
+#include <string>
+#include <unordered_map>
 #include <cstring>
 #include <numeric>
 
@@ -151,30 +153,58 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
     L.s = S;
     L.tile = tile;
     L.ntiles = N > 0 ? (N + tile - 1) / tile : 0;
-    L.kind = net.kind;
-    L.Cm.assign(N, 0.f); L.CE.assign(N, 0.f); L.CI.assign(N, 0.f);
-    L.C_E.assign(N, 0.f); L.C_I.assign(N, 0.f); L.v_tmp.assign(N, 0.f);
-    L.v_thresh.assign(N, 0.f); L.v_reset.assign(N, 0.f); L.v_init.assign(N, 0.f);
-    L.refrac_time.assign(N, 0);
+    L.ncls.assign(N, 0u);
+    L.lifc.clear();
+    L.v_tmp.assign(N, 0.f);
+    L.v_init.assign(N, 0.f);
     L.arr_cur0.assign(N, 0);
     L.arr_end.assign(N, 0);
     L.bern_p.assign(N, 0u);
     L.bern_seed.assign(N, 0ull);
     L.bern_key.assign(N, 0u);
+    /* LIF parameter sets: the constants shared by neurons are stored once
+     * (consecutive neurons usually share them: the previous neuron's set is
+     * tried first) */
+    std::unordered_map<std::string, uint32_t> set_of;
+    uint32_t prev_set = 0xFFFFFFFFu;
     for (int32_t i = 0; i < N; i++) {
         if (net.kind[i] == BSIM_KIND_LIF) {
             float c[9];
-            lif_constants(&net.lif[(size_t)i * kLifParams], dt, c, &L.refrac_time[i]);
-            L.Cm[i] = c[0]; L.CE[i] = c[1]; L.CI[i] = c[2]; L.C_E[i] = c[3]; L.C_I[i] = c[4];
-            L.v_tmp[i] = c[5]; L.v_thresh[i] = c[6]; L.v_reset[i] = c[7]; L.v_init[i] = c[8];
+            bsim_lifc k;
+            lif_constants(&net.lif[(size_t)i * kLifParams], dt, c, &k.refrac_time);
+            k.Cm = c[0]; k.CE = c[1]; k.CI = c[2]; k.C_E = c[3]; k.C_I = c[4];
+            k.v_thresh = c[6]; k.v_reset = c[7];
+            L.v_tmp[i] = c[5];
+            L.v_init[i] = c[8];
+            uint32_t set;
+            if (prev_set != 0xFFFFFFFFu && memcmp(&L.lifc[prev_set], &k, sizeof(k)) == 0) {
+                set = prev_set;
+            } else {
+                const std::string key(reinterpret_cast<const char *>(&k), sizeof(k));
+                auto it = set_of.find(key);
+                if (it == set_of.end()) {
+                    if (L.lifc.size() > BSIM_CLS_SET_MASK) {
+                        err = "too many distinct LIF parameter sets";
+                        return -1;
+                    }
+                    it = set_of.emplace(key, (uint32_t)L.lifc.size()).first;
+                    L.lifc.push_back(k);
+                }
+                set = it->second;
+            }
+            prev_set = set;
+            L.ncls[i] = ((uint32_t)BSIM_KIND_LIF << 30) | set;
         } else if (net.kind[i] == BSIM_KIND_ARRAY) {
+            L.ncls[i] = (uint32_t)BSIM_KIND_ARRAY << 30;
             L.arr_cur0[i] = (int32_t)net.arr_off[i];
             L.arr_end[i] = (int32_t)net.arr_off[i + 1];
         } else if (net.kind[i] == BSIM_KIND_BERNOULLI) {
+            L.ncls[i] = (uint32_t)BSIM_KIND_BERNOULLI << 30;
             L.bern_p[i] = net.bern_p[i];
             L.bern_seed[i] = net.bern_seed[i];
             L.bern_key[i] = net.bern_key[i];
         } else if (net.kind[i] == BSIM_KIND_POISSON) {
+            L.ncls[i] = (uint32_t)BSIM_KIND_POISSON << 30;
             L.bern_seed[i] = net.bern_seed[i];
             L.bern_key[i] = net.bern_key[i];
         } else {

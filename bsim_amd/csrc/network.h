@@ -53,9 +53,10 @@ struct HostNet {
 struct Layout {
     int32_t n = 0, tile = 0, ntiles = 0, dmax = 1;
     int64_t s = 0;
-    std::vector<int8_t> kind;
-    std::vector<float> Cm, CE, CI, C_E, C_I, v_tmp, v_thresh, v_reset, v_init;
-    std::vector<int32_t> refrac_time, arr_cur0, arr_end, arr_t;
+    std::vector<uint32_t> ncls;              /* kind << 30 | LIF parameter set */
+    std::vector<bsim_lifc> lifc;             /* the distinct LIF parameter sets */
+    std::vector<float> v_tmp, v_init;
+    std::vector<int32_t> arr_cur0, arr_end, arr_t;
     std::vector<uint32_t> bern_p;
     std::vector<uint64_t> bern_seed;
     std::vector<uint32_t> bern_key;

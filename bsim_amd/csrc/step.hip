@@ -600,7 +600,8 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
             bool fired = false;
             uint64_t h = 0;
             if (valid) {
-                const int8_t kind = d.kind[nid];
+                const uint32_t cls = d.ncls[nid];
+                const uint32_t kind = cls >> 30;
                 if (kind == BSIM_KIND_LIF) {
                     const float inE = acc.e[j];
                     const float inI = acc.i[j];
@@ -608,14 +609,15 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
                     if (rs > 0) {
                         d.refrac[nid] = rs - 1;
                     } else {
+                        const bsim_lifc c = d.lifc[cls & BSIM_CLS_SET_MASK];
                         float v = d.v[nid], ie = d.ie[nid], ii = d.ii[nid];
-                        v = d.Cm[nid] * v + d.v_tmp[nid] + ie * d.C_E[nid] + ii * d.C_I[nid];
-                        ie = d.CE[nid] * ie;
-                        ii = d.CI[nid] * ii;
-                        if (v >= d.v_thresh[nid]) {
+                        v = c.Cm * v + d.v_tmp[nid] + ie * c.C_E + ii * c.C_I;
+                        ie = c.CE * ie;
+                        ii = c.CI * ii;
+                        if (v >= c.v_thresh) {
                             fired = true;
-                            d.refrac[nid] = d.refrac_time[nid] - 1;
-                            v = d.v_reset[nid];
+                            d.refrac[nid] = c.refrac_time - 1;
+                            v = c.v_reset;
                         } else {
                             if (d.in_sum) d.in_sum[nid] = d.in_sum[nid] + (inE + inI);
                             ie += inE;
