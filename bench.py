@@ -506,9 +506,12 @@ def main():
     sim.step(a.warmup)
     sim.sync()
     start = a.preroll + a.warmup
-    # the first min(K, --cpu-steps) timed cycles are the CPU baseline's window;
-    # their deliveries are read at a stream synchronisation inside the timed
-    # region (one extra host wait when K is larger)
+    # the first min(K, --cpu-steps) timed launches are the CPU baseline's
+    # window; their deliveries are read at a stream synchronisation inside the
+    # timed region (one extra host wait when K is larger).  Launch L (counted
+    # from 1) performs the delivery phase of cycle L - 2 and the update of
+    # cycle L - 1, so timed launches start+1 .. start+K deliver cycles
+    # start-1 .. start+K-2: the oracle times and counts those cycles.
     k_cpu = 0 if a.no_cpu else min(a.steps, a.cpu_steps)
     ev0 = sim.events()
     t0 = time.perf_counter()
@@ -550,11 +553,12 @@ def main():
     cpu_ref = None
     if not a.no_cpu:
         threads = a.cpu_threads or min(os.cpu_count() or 1, 16)
-        v, ev, secs = cpu_baseline(spec, start, k_cpu, threads)
+        c0 = max(start - 1, 0)
+        v, ev, secs = cpu_baseline(spec, c0, k_cpu, threads)
         cpu = {"value": v, "unit": "synaptic events/s", "cores": threads, "kind": "port",
-               "sample": f"oracle CSR form (OpenMP), cycles {start}..{start + k_cpu} of the same "
-                         f"network = the GPU's first {k_cpu} timed cycles ({ev} events in "
-                         f"{secs:.3f}s; cycles 0..{start} run untimed first)",
+               "sample": f"oracle CSR form (OpenMP), cycles {c0}..{c0 + k_cpu} of the same "
+                         f"network = the deliveries of the GPU's first {k_cpu} timed launches "
+                         f"({ev} events in {secs:.3f}s; cycles 0..{c0} run untimed first)",
                "events_match": bool(ev == ev_cpu_window), "gpu_over_cpu": value / v,
                "host": host_info()}
         print(f"[bench] cpu baseline {v:.3g} events/s on {threads} threads over the same "
@@ -580,6 +584,7 @@ def main():
         "config": {"workload": desc, "name": a.workload, "neurons": n_neurons,
                    "synapses": n_syn, "dt_s": netspec.DT, "tile": info["tile"],
                    "preroll": a.preroll, "timed_cycles": [start, start + a.steps],
+                   "timed_delivery_cycles": [start - 1, start + a.steps - 1],
                    "events_per_cycle": events / max(a.steps, 1),
                    "simulated_s": a.steps * netspec.DT, "parallelism": "single GPU"},
         "roofline": {"bound": "hbm", "achieved": achieved, "peak": HBM_PEAK_GBS,

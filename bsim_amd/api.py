@@ -63,6 +63,7 @@ _SIGS = {
     "bsim_sim_destroy": (None, [P]),
     "bsim_net_gen_standard": (C.c_int64, [P, P]),
     "bsim_net_kinds": (C.c_int32, [P, P]),
+    "bsim_sim_create_chains": (P, [P, C.c_int32, C.c_float, P, C.c_int64, P]),
     "bsim_net_export_neurons": (C.c_int32, [P, P, P, P, P]),
     "bsim_net_export_synapses": (C.c_int64, [P, C.c_int64, C.c_int64, P, P, P, P, P]),
     "bsim_net_lif_constants": (C.c_int32, [P, C.c_float, P, P]),
@@ -375,6 +376,30 @@ class Simulation:
         self.n = net.n
         self.n_rec = len(self._rec)
         self.rec_steps = rec_steps
+
+    @classmethod
+    def from_chains(cls, kwargs_list, dt, tile=0, rec_steps=0, part_synapses=int(1.5e9),
+                    device=0):
+        """One simulation of several sweep networks built part by part
+        (bsim_sim_create_chains); returns (simulation, first ids)."""
+        n = len(kwargs_list)
+        arr = (ChainParams * max(n, 1))()
+        for i, kw in enumerate(kwargs_list):
+            Network.chain_params(out=arr[i], **kw)
+        first = np.zeros(max(n, 1), np.int32)
+        self = cls.__new__(cls)
+        self.dt = np.float32(dt)
+        self._rec = np.zeros(0, np.int32)
+        cfg = SimConfig(device, tile, rec_steps, 0, None)
+        self._h = lib().bsim_sim_create_chains(arr, n, float(self.dt), C.byref(cfg),
+                                               int(part_synapses), _p(first))
+        if not self._h:
+            _err(-1, "bsim_sim_create_chains")
+        info = self.info()
+        self.n = info["n"]
+        self.n_rec = 0
+        self.rec_steps = rec_steps
+        return self, first[:n].astype(np.int64)
 
     def close(self):
         h = getattr(self, "_h", None)

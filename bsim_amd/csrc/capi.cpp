@@ -263,16 +263,34 @@ int32_t bsim_net_layout(const bsim_network *net, float dt, int32_t tile, int32_t
 
 }  // extern "C"
 
+/* Releases a simulation: waits for its stream, destroys the captured graph
+ * before the memory its launches use, frees, destroys the stream.  Every step
+ * is attempted; the first failure is kept as the thread's error message. */
 static void sim_free(bsim_sim *sim)
 {
     if (!sim) return;
-    hipSetDevice(sim->device);
-    if (sim->stream) hipStreamSynchronize(sim->stream);
-    if (sim->gexec) hipGraphExecDestroy(sim->gexec);
-    if (sim->graph) hipGraphDestroy(sim->graph);
-    for (void *p : sim->allocs) hipFree(p);
-    if (sim->stream) hipStreamDestroy(sim->stream);
+    hipError_t first = hipSuccess;
+    auto keep = [&first](hipError_t e) {
+        if (e != hipSuccess && first == hipSuccess) first = e;
+    };
+    keep(hipSetDevice(sim->device));
+    if (sim->stream) keep(hipStreamSynchronize(sim->stream));
+    if (sim->gexec) keep(hipGraphExecDestroy(sim->gexec));
+    if (sim->graph) keep(hipGraphDestroy(sim->graph));
+    sim->gexec = nullptr;
+    sim->graph = nullptr;
+    for (void *p : sim->allocs) keep(hipFree(p));
+    sim->allocs.clear();
+    if (sim->stream) keep(hipStreamDestroy(sim->stream));
+    if (first != hipSuccess) fail(BSIM_E_HIP, std::string("bsim_sim_destroy: ") + hipGetErrorString(first));
     delete sim;
+}
+
+/* Makes the simulation's device current (0, or the failure code). */
+static int32_t use_device(const bsim_sim *sim)
+{
+    const hipError_t e = hipSetDevice(sim->device);
+    return e == hipSuccess ? 0 : fail(BSIM_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
 }
 
 template <class T>
@@ -300,6 +318,29 @@ static T *dev_upload(bsim_sim *sim, const std::vector<T> &v, hipError_t &e)
 
 extern "C" {
 
+}  // extern "C"
+
+/* Checks the simulation config and makes its device current (0, or a
+ * failure code). */
+static int32_t check_config(const bsim_sim_config &c, int32_t N, const char *who)
+{
+    if (c.rec_steps < 0 || c.n_rec < 0 || (c.n_rec > 0 && !c.rec_ids))
+        return fail(BSIM_E_ARG, std::string(who) + ": bad recording configuration");
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return fail(BSIM_E_HIP, "no HIP device available");
+    if (c.device < 0 || c.device >= ndev) return fail(BSIM_E_ARG, "device ordinal out of range");
+    const hipError_t e = hipSetDevice(c.device);
+    if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
+    for (int32_t r = 0; r < c.n_rec; r++)
+        if (c.rec_ids[r] < 0 || c.rec_ids[r] >= N) return fail(BSIM_E_ARG, "recorded neuron id out of range");
+    return 0;
+}
+
+static bsim_sim *sim_from_layout(bsim::Layout &L, const bsim_sim_config &c, bsim_syn *dev_syn,
+                                 int64_t dev_syn_bytes);
+
+extern "C" {
+
 bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_config *cfg)
 {
     if (!net || !(dt > 0)) {
@@ -308,32 +349,9 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
     }
     bsim_sim_config c{};
     if (cfg) c = *cfg;
-    if (c.rec_steps < 0 || c.n_rec < 0 || (c.n_rec > 0 && !c.rec_ids)) {
-        fail(BSIM_E_ARG, "bsim_sim_create: bad recording configuration");
-        return nullptr;
-    }
-    int ndev = 0;
-    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) {
-        fail(BSIM_E_HIP, "no HIP device available");
-        return nullptr;
-    }
-    if (c.device < 0 || c.device >= ndev) {
-        fail(BSIM_E_ARG, "device ordinal out of range");
-        return nullptr;
-    }
-    hipError_t e = hipSetDevice(c.device);
-    if (e != hipSuccess) {
-        fail(BSIM_E_HIP, std::string("hipSetDevice: ") + hipGetErrorString(e));
-        return nullptr;
-    }
     const int32_t N = net->net.n();
+    if (check_config(c, N, "bsim_sim_create") != 0) return nullptr;
     const int32_t tile = c.tile > 0 ? c.tile : bsim::default_tile(bsim::max_delay_steps(net->net, dt));
-    for (int32_t r = 0; r < c.n_rec; r++) {
-        if (c.rec_ids[r] < 0 || c.rec_ids[r] >= N) {
-            fail(BSIM_E_ARG, "recorded neuron id out of range");
-            return nullptr;
-        }
-    }
     bsim::Layout L;
     std::string err;
     try {
@@ -345,16 +363,97 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
         fail(BSIM_E_NOMEM, "out of host memory while building the layout");
         return nullptr;
     }
+    return sim_from_layout(L, c, nullptr, 0);
+}
+
+bsim_sim *bsim_sim_create_chains(const bsim_chain_params *p, int32_t count, float dt,
+                                 const bsim_sim_config *cfg, int64_t part_synapses,
+                                 int32_t *first_ids)
+{
+    if (!p || count <= 0 || !(dt > 0) || part_synapses <= 0) {
+        fail(BSIM_E_ARG, "bsim_sim_create_chains: bad argument");
+        return nullptr;
+    }
+    bsim_sim_config c{};
+    if (cfg) c = *cfg;
+    int64_t nn = 0, ns = 0;
+    int32_t dmax = 1;
+    if (bsim::chains_size(p, count, &nn, &ns, &dmax) != 0) {
+        fail(BSIM_E_ARG, "bsim_sim_create_chains: invalid chain parameters");
+        return nullptr;
+    }
+    const int32_t tile = c.tile > 0 ? c.tile : bsim::default_tile(dmax);
+    /* recorded ids are checked against the laid-out network below */
+    if (check_config(c, 0x7FFFFFFF, "bsim_sim_create_chains") != 0) return nullptr;
+    /* the packed synapses go straight to the device, part by part */
+    bsim_syn *dsyn = nullptr;
+    const int64_t sbytes = std::max<int64_t>(ns, 1) * (int64_t)sizeof(bsim_syn);
+    hipError_t e = hipMalloc((void **)&dsyn, (size_t)sbytes);
+    if (e != hipSuccess) {
+        fail(e == hipErrorOutOfMemory ? BSIM_E_NOMEM : BSIM_E_HIP,
+             std::string("bsim_sim_create_chains: ") + hipGetErrorString(e));
+        return nullptr;
+    }
+    bsim::Layout L;
+    std::string err;
+    int rc = 0;
+    try {
+        auto upload = [dsyn](const bsim::Layout &part, int64_t s0) -> int {
+            std::vector<bsim_syn> packed(part.syn_tgt.size());
+#pragma omp parallel for schedule(static)
+            for (int64_t i = 0; i < (int64_t)packed.size(); i++) {
+                packed[i].tgt = part.syn_tgt[i];
+                memcpy(&packed[i].w, &part.syn_w[i], 4);
+            }
+            if (packed.empty()) return 0;
+            const hipError_t x = hipMemcpy(dsyn + s0, packed.data(), packed.size() * sizeof(bsim_syn),
+                                           hipMemcpyHostToDevice);
+            return x == hipSuccess ? 0 : fail(BSIM_E_HIP, std::string("synapse upload: ") + hipGetErrorString(x));
+        };
+        rc = bsim::build_chains_layout(p, count, dt, tile, dmax, part_synapses, L, first_ids, err, upload);
+    } catch (const std::bad_alloc &) {
+        rc = -1;
+        err = "out of host memory while building the layout";
+    }
+    if (rc != 0) {
+        if (!err.empty()) fail(BSIM_E_NETWORK, err);
+        (void)hipFree(dsyn);
+        return nullptr;
+    }
+    for (int32_t r = 0; r < c.n_rec; r++)
+        if (c.rec_ids[r] < 0 || c.rec_ids[r] >= L.n) {
+            (void)hipFree(dsyn);
+            fail(BSIM_E_ARG, "recorded neuron id out of range");
+            return nullptr;
+        }
+    return sim_from_layout(L, c, dsyn, sbytes);
+}
+
+}  // extern "C"
+
+/* Uploads a layout and allocates the step state; dev_syn, when not null, is
+ * the packed synapse array already on the device (adopted by the
+ * simulation, freed with it). */
+static bsim_sim *sim_from_layout(bsim::Layout &L, const bsim_sim_config &c, bsim_syn *dev_syn,
+                                 int64_t dev_syn_bytes)
+{
+    hipError_t e = hipSuccess;
     bsim_sim *sim = new (std::nothrow) bsim_sim();
     if (!sim) {
+        if (dev_syn) (void)hipFree(dev_syn);
         fail(BSIM_E_NOMEM, "out of host memory");
         return nullptr;
+    }
+    if (dev_syn) {
+        sim->allocs.push_back(dev_syn);
+        sim->device_bytes += dev_syn_bytes;
     }
     sim->device = c.device;
     e = hipStreamCreateWithFlags(&sim->stream, hipStreamNonBlocking);
     if (e != hipSuccess) {
         fail(BSIM_E_HIP, std::string("hipStreamCreate: ") + hipGetErrorString(e));
-        delete sim;
+        sim->stream = nullptr;
+        sim_free(sim);
         return nullptr;
     }
     bsim_dev &d = sim->d;
@@ -384,7 +483,9 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
     UP(tile_nsyn, L.tile_nsyn);
     UP(tile_res, L.tile_res);
     d.nslots = L.nslots;
-    {
+    if (dev_syn) {
+        d.syn = dev_syn;
+    } else {
         /* targets and weights packed into one 8-byte word per synapse */
         std::vector<bsim_syn> packed(L.syn_tgt.size());
 #pragma omp parallel for schedule(static)
@@ -423,7 +524,7 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
         if (!(d.ev_count = dev_alloc<int64_t>(sim, nt, e))) goto hip_fail;
         if (!(sim->ev_fin = dev_alloc<int64_t>(sim, nt, e))) goto hip_fail;
         if (!(d.clock = dev_alloc<int32_t>(sim, 4, e))) goto hip_fail;
-        hipMemsetAsync(d.clock, 0, 4 * sizeof(int32_t), sim->stream);
+        if ((e = hipMemsetAsync(d.clock, 0, 4 * sizeof(int32_t), sim->stream)) != hipSuccess) goto hip_fail;
         {
             /* until the first ranking: the tiles of each residue in order */
             std::vector<int32_t> perm((size_t)(L.nslots > 0 ? L.nslots : 1), -1);
@@ -436,27 +537,27 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
             if (e != hipSuccess) goto hip_fail;
         }
         if (!(d.tile_load = dev_alloc<uint64_t>(sim, nt, e))) goto hip_fail;
-        hipMemsetAsync(d.tile_load, 0, nt * sizeof(uint64_t), sim->stream);
+        if ((e = hipMemsetAsync(d.tile_load, 0, nt * sizeof(uint64_t), sim->stream)) != hipSuccess) goto hip_fail;
         if (!(d.tile_cls = dev_alloc<uint8_t>(sim, nt, e))) goto hip_fail;
         const size_t ns = (size_t)(d.nsrc > 0 ? d.nsrc : 1);
         if (!(d.src_count = dev_alloc<uint32_t>(sim, ns, e))) goto hip_fail;
-        hipMemsetAsync(d.src_count, 0, ns * sizeof(uint32_t), sim->stream);
+        if ((e = hipMemsetAsync(d.src_count, 0, ns * sizeof(uint32_t), sim->stream)) != hipSuccess) goto hip_fail;
         if (!(d.src_list = dev_alloc<uint32_t>(sim, (size_t)(L.src_list_len > 0 ? L.src_list_len : 1), e)))
             goto hip_fail;
-        hipMemsetAsync(d.ie, 0, n * 4, sim->stream);
-        hipMemsetAsync(d.ii, 0, n * 4, sim->stream);
-        hipMemsetAsync(d.refrac, 0, n * 4, sim->stream);
-        hipMemsetAsync(d.fire_count, 0, n * 4, sim->stream);
-        hipMemsetAsync(d.hist, 0, 2 * n * 8, sim->stream);
-        hipMemsetAsync(d.active, 0, 2 * nw * 8, sim->stream);
-        hipMemsetAsync(d.ev_count, 0, nt * 8, sim->stream);
-        hipMemsetAsync(sim->ev_fin, 0, nt * 8, sim->stream);
+        if ((e = hipMemsetAsync(d.ie, 0, n * 4, sim->stream)) != hipSuccess) goto hip_fail;
+        if ((e = hipMemsetAsync(d.ii, 0, n * 4, sim->stream)) != hipSuccess) goto hip_fail;
+        if ((e = hipMemsetAsync(d.refrac, 0, n * 4, sim->stream)) != hipSuccess) goto hip_fail;
+        if ((e = hipMemsetAsync(d.fire_count, 0, n * 4, sim->stream)) != hipSuccess) goto hip_fail;
+        if ((e = hipMemsetAsync(d.hist, 0, 2 * n * 8, sim->stream)) != hipSuccess) goto hip_fail;
+        if ((e = hipMemsetAsync(d.active, 0, 2 * nw * 8, sim->stream)) != hipSuccess) goto hip_fail;
+        if ((e = hipMemsetAsync(d.ev_count, 0, nt * 8, sim->stream)) != hipSuccess) goto hip_fail;
+        if ((e = hipMemsetAsync(sim->ev_fin, 0, nt * 8, sim->stream)) != hipSuccess) goto hip_fail;
         if (c.rec_steps > 0) {
             d.rec_steps = c.rec_steps;
             d.rec_t0 = 0;
             const size_t rb = (size_t)c.rec_steps * (size_t)(d.nwords > 0 ? d.nwords : 1);
             if (!(d.raster = dev_alloc<uint64_t>(sim, rb, e))) goto hip_fail;
-            hipMemsetAsync(d.raster, 0, rb * 8, sim->stream);
+            if ((e = hipMemsetAsync(d.raster, 0, rb * 8, sim->stream)) != hipSuccess) goto hip_fail;
             if (c.n_rec > 0) {
                 /* one device column per distinct neuron; repeated ids share it */
                 std::vector<int32_t> idx(L.n, -1);
@@ -472,7 +573,7 @@ bsim_sim *bsim_sim_create(const bsim_network *net, float dt, const bsim_sim_conf
                 d.n_rec = ncol;
                 const size_t vb = (size_t)c.rec_steps * (size_t)ncol;
                 if (!(d.rec_v = dev_alloc<float>(sim, vb, e))) goto hip_fail;
-                hipMemsetAsync(d.rec_v, 0, vb * 4, sim->stream);
+                if ((e = hipMemsetAsync(d.rec_v, 0, vb * 4, sim->stream)) != hipSuccess) goto hip_fail;
             }
         }
         sim->rec_steps = c.rec_steps;
@@ -487,8 +588,6 @@ hip_fail:
     sim_free(sim);
     return nullptr;
 }
-
-}  // extern "C"
 
 /* Cycles per captured graph: runs of this many cycles are replayed from one
  * hipGraph of identical clocked launches (the step kernel reads and advances
@@ -529,7 +628,7 @@ int32_t bsim_sim_step(bsim_sim *sim, int32_t n)
 {
     if (!sim || n < 0) return fail(BSIM_E_ARG, "bsim_sim_step: bad argument");
     if ((int64_t)sim->cycle + n > 0x7FFFFFFF) return fail(BSIM_E_ARG, "bsim_sim_step: cycle overflow");
-    hipSetDevice(sim->device);
+    if (int32_t rc = use_device(sim)) return rc;
     hipError_t e = hipSuccess;
     if (n >= kGraphCycles && !sim->gexec) {
         e = build_graph(sim);
@@ -553,7 +652,7 @@ int32_t bsim_sim_step(bsim_sim *sim, int32_t n)
 int32_t bsim_sim_time_steps(bsim_sim *sim, int32_t n, double *step_us, double *rows_us)
 {
     if (!sim || n <= 0) return fail(BSIM_E_ARG, "bsim_sim_time_steps: bad argument");
-    hipSetDevice(sim->device);
+    if (int32_t rc = use_device(sim)) return rc;
     std::vector<hipEvent_t> ev((size_t)n * 3, nullptr);
     hipError_t e = hipSuccess;
     for (size_t i = 0; i < ev.size() && e == hipSuccess; i++) e = hipEventCreate(&ev[i]);
@@ -586,11 +685,12 @@ int32_t bsim_sim_finish(bsim_sim *sim)
 {
     if (!sim) return fail(BSIM_E_ARG, "bad argument");
     if (sim->cycle == 0 || sim->finished_at == sim->cycle) return 0;
-    hipSetDevice(sim->device);
+    if (int32_t rc = use_device(sim)) return rc;
     bsim_dev d = sim->d;
     d.ev_count = sim->ev_fin;
-    hipMemsetAsync(sim->ev_fin, 0, (size_t)(d.ntiles > 0 ? d.ntiles : 1) * 8, sim->stream);
-    hipError_t e = bsim_launch_step(&d, sim->cycle, 1, sim->stream);
+    hipError_t e = hipMemsetAsync(sim->ev_fin, 0, (size_t)(d.ntiles > 0 ? d.ntiles : 1) * 8, sim->stream);
+    if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("finish: ") + hipGetErrorString(e));
+    e = bsim_launch_step(&d, sim->cycle, 1, sim->stream);
     if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("finish launch: ") + hipGetErrorString(e));
     sim->finished_at = sim->cycle;
     return 0;
@@ -599,7 +699,7 @@ int32_t bsim_sim_finish(bsim_sim *sim)
 int32_t bsim_sim_sync(bsim_sim *sim)
 {
     if (!sim) return fail(BSIM_E_ARG, "bad argument");
-    hipSetDevice(sim->device);
+    if (int32_t rc = use_device(sim)) return rc;
     hipError_t e = hipStreamSynchronize(sim->stream);
     if (e != hipSuccess) return fail(BSIM_E_HIP, std::string("stream: ") + hipGetErrorString(e));
     return 0;
@@ -625,7 +725,7 @@ int64_t bsim_sim_read_raster(bsim_sim *sim, int32_t *counts, int32_t *ids, int64
 {
     if (!sim) return fail(BSIM_E_ARG, "bad argument");
     if (!sim->d.raster) return fail(BSIM_E_STATE, "raster recording was not enabled");
-    hipSetDevice(sim->device);
+    if (int32_t rc = use_device(sim)) return rc;
     const int32_t steps = sim->cycle < sim->rec_steps ? sim->cycle : sim->rec_steps;
     std::vector<uint64_t> bits((size_t)steps * sim->d.nwords);
     hipError_t e = fetch(sim, bits.data(), sim->d.raster, bits.size());
@@ -655,7 +755,7 @@ int32_t bsim_sim_read_trace(bsim_sim *sim, float *v)
 {
     if (!sim || !v) return fail(BSIM_E_ARG, "bad argument");
     if (!sim->d.rec_v) return fail(BSIM_E_STATE, "no neuron is traced");
-    hipSetDevice(sim->device);
+    if (int32_t rc = use_device(sim)) return rc;
     const int32_t steps = sim->cycle < sim->rec_steps ? sim->cycle : sim->rec_steps;
     const int32_t ncol = sim->d.n_rec;
     std::vector<float> cols((size_t)steps * (size_t)ncol);
@@ -692,8 +792,9 @@ int32_t bsim_sim_enable_state_trace(bsim_sim *sim)
     if (sim->gexec) {
         if ((e = hipStreamSynchronize(sim->stream)) != hipSuccess)
             return fail(BSIM_E_HIP, std::string("state trace: ") + hipGetErrorString(e));
-        hipGraphExecDestroy(sim->gexec);
-        hipGraphDestroy(sim->graph);
+        if ((e = hipGraphExecDestroy(sim->gexec)) != hipSuccess ||
+            (e = hipGraphDestroy(sim->graph)) != hipSuccess)
+            return fail(BSIM_E_HIP, std::string("state trace: ") + hipGetErrorString(e));
         sim->gexec = nullptr;
         sim->graph = nullptr;
     }
@@ -726,7 +827,7 @@ int32_t bsim_sim_read_state(bsim_sim *sim, float *v, float *ie, float *ii, int32
                             int32_t *fire_count)
 {
     if (!sim) return fail(BSIM_E_ARG, "bad argument");
-    hipSetDevice(sim->device);
+    if (int32_t rc = use_device(sim)) return rc;
     const size_t n = (size_t)sim->d.n;
     hipError_t e;
     if ((e = fetch(sim, v, sim->d.v, n)) != hipSuccess ||
@@ -742,7 +843,7 @@ int32_t bsim_sim_enable_input_log(bsim_sim *sim)
 {
     if (!sim) return fail(BSIM_E_ARG, "bad argument");
     if (sim->log_in) return 0;
-    hipSetDevice(sim->device);
+    if (int32_t rc = use_device(sim)) return rc;
     const size_t n = (size_t)(sim->d.n > 0 ? sim->d.n : 1);
     hipError_t e = hipSuccess;
     float *in = dev_alloc<float>(sim, n, e);
@@ -759,9 +860,11 @@ int32_t bsim_sim_enable_input_log(bsim_sim *sim)
     sim->d.slot_count = sim->s > 0 ? slot : nullptr;
     /* the captured launches hold the kernel arguments: capture again */
     if (sim->gexec) {
-        hipStreamSynchronize(sim->stream);
-        hipGraphExecDestroy(sim->gexec);
-        hipGraphDestroy(sim->graph);
+        hipError_t x;
+        if ((x = hipStreamSynchronize(sim->stream)) != hipSuccess ||
+            (x = hipGraphExecDestroy(sim->gexec)) != hipSuccess ||
+            (x = hipGraphDestroy(sim->graph)) != hipSuccess)
+            return fail(BSIM_E_HIP, std::string("input log: ") + hipGetErrorString(x));
         sim->gexec = nullptr;
         sim->graph = nullptr;
     }
@@ -772,7 +875,7 @@ int32_t bsim_sim_read_input_log(bsim_sim *sim, float *in_sum, int32_t *slot_coun
 {
     if (!sim) return fail(BSIM_E_ARG, "bad argument");
     if (!sim->log_in) return fail(BSIM_E_STATE, "input log not enabled (bsim_sim_enable_input_log)");
-    hipSetDevice(sim->device);
+    if (int32_t rc = use_device(sim)) return rc;
     const size_t n = (size_t)sim->d.n;
     hipError_t e;
     if ((e = fetch(sim, in_sum, sim->log_in, n)) != hipSuccess ||
@@ -784,7 +887,7 @@ int32_t bsim_sim_read_input_log(bsim_sim *sim, float *in_sum, int32_t *slot_coun
 int64_t bsim_sim_events(bsim_sim *sim)
 {
     if (!sim) return fail(BSIM_E_ARG, "bad argument");
-    hipSetDevice(sim->device);
+    if (int32_t rc = use_device(sim)) return rc;
     const size_t nt = (size_t)sim->d.ntiles;
     std::vector<int64_t> a(nt), b(nt);
     hipError_t e = fetch(sim, a.data(), sim->d.ev_count, nt);

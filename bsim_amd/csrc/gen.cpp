@@ -24,6 +24,7 @@
 #include <stdint.h>
 
 #include <algorithm>
+#include <functional>
 #include <string>
 #include <vector>
 
@@ -402,6 +403,86 @@ extern "C" int32_t bsim_net_gen_chains(bsim_network *netp, const bsim_chain_para
         return BSIM_E_NOMEM;
     }
 }
+
+namespace bsim {
+
+int chains_size(const bsim_chain_params *p, int32_t count, int64_t *neurons, int64_t *synapses,
+                int32_t *dmax)
+{
+    int64_t n = 0, s = 0;
+    int32_t d = 1;
+    for (int32_t c = 0; c < count; c++) {
+        if (!chain_valid(p[c])) return -1;
+        const ChainPlan pl = plan_chain(p[c], 0, 0);
+        n += pl.nn;
+        s += pl.nsyn;
+        d = std::max(d, p[c].dmax);
+    }
+    *neurons = n;
+    *synapses = s;
+    *dmax = d;
+    return 0;
+}
+
+int build_chains_layout(const bsim_chain_params *p, int32_t count, float dt, int32_t tile,
+                        int32_t dmax, int64_t part_synapses, Layout &out, int32_t *first_ids,
+                        std::string &err,
+                        const std::function<int(const Layout &, int64_t)> &upload)
+{
+    out = Layout();
+    int32_t c0 = 0;
+    while (c0 < count) {
+        /* the next part: chains while their synapses fit part_synapses (at
+         * least one chain) */
+        int32_t c1 = c0;
+        int64_t syn = 0;
+        while (c1 < count) {
+            const int64_t cs = plan_chain(p[c1], 0, 0).nsyn;
+            if (c1 > c0 && syn + cs > part_synapses) break;
+            syn += cs;
+            c1++;
+        }
+        bsim_network part;
+        std::vector<int32_t> first((size_t)(c1 - c0));
+        if (bsim_net_gen_chains(&part, p + c0, c1 - c0, first.data()) < 0) {
+            err = g_bsim_err;
+            return -1;
+        }
+        /* inert neurons (array sources without spikes) pad the part to a
+         * whole number of tiles, so that the next part starts on a tile */
+        HostNet &h = part.net;
+        const int32_t pad = (int32_t)((tile - h.n() % tile) % tile);
+        for (int32_t i = 0; i < pad; i++) {
+            h.kind.push_back((int8_t)BSIM_KIND_ARRAY);
+            h.lif.insert(h.lif.end(), (size_t)kLifParams, 0.0f);
+            h.arr_off.push_back(h.arr_off.back());
+            h.bern_p.push_back(0u);
+            h.bern_seed.push_back(0ull);
+            h.bern_key.push_back(0u);
+        }
+        if ((int64_t)out.n + h.n() > 0x7FFFFFFF) {
+            err = "bsim_sim_create_chains: more than 2^31 - 1 neurons";
+            return -1;
+        }
+        Layout pl;
+        if (build_layout(h, dt, tile, pl, err, dmax) != 0) return -1;
+        part = bsim_network();          /* the host network is no longer needed */
+        if (upload(pl, out.s) != 0) {
+            err.clear();                /* the upload set the error message */
+            return -1;
+        }
+        std::vector<uint16_t>().swap(pl.syn_tgt);
+        std::vector<float>().swap(pl.syn_w);
+        const int32_t base = out.n;
+        if (append_layout(out, pl, err) != 0) return -1;
+        if (first_ids)
+            for (int32_t c = c0; c < c1; c++) first_ids[c] = base + first[(size_t)(c - c0)];
+        c0 = c1;
+    }
+    return 0;
+}
+
+}  // namespace bsim
 
 extern "C" int32_t bsim_net_gen_chain(bsim_network *netp, const bsim_chain_params *gp)
 {

@@ -135,7 +135,8 @@ static int reset_synapse(const HostNet &net, int64_t k, int32_t N, float dt, flo
     return 0;
 }
 
-int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::string &err)
+int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::string &err,
+                 int32_t dmax_min)
 {
     const int32_t N = net.n();
     const int64_t S = net.s();
@@ -270,6 +271,11 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
         int32_t st;
         return reset_synapse(net, first_bad, N, dt, &wk, &st, &err);
     }
+    if (dmax_min > BSIM_MAX_DELAY) {
+        err = "delay ring deeper than " + std::to_string(BSIM_MAX_DELAY) + " steps";
+        return -4;
+    }
+    dmax = std::max(dmax, dmax_min);
     L.dmax = dmax;
 
     /* bucket synapses by target tile, keeping creation order: a parallel
@@ -599,6 +605,111 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
                 L.chunk_wrow[x] = L.rng[ri].wrow;
             }
     }
+    return 0;
+}
+
+int append_layout(Layout &A, Layout &B, std::string &err)
+{
+    if (A.ntiles > 0 && (A.tile != B.tile || A.dmax != B.dmax)) {
+        err = "append_layout: parts differ in tile size or delay ring depth";
+        return -1;
+    }
+    if (A.ntiles > 0 && A.n != A.ntiles * A.tile) {
+        err = "append_layout: the first part does not end on a tile boundary";
+        return -1;
+    }
+    if ((int64_t)A.n + B.n > 0x7FFFFFFF) {
+        err = "append_layout: more than 2^31 - 1 neurons";
+        return -1;
+    }
+    if (A.ntiles == 0) {
+        const int32_t pn = A.n;
+        A = std::move(B);
+        if (pn != 0) {
+            err = "append_layout: neurons without tiles";
+            return -1;
+        }
+        return 0;
+    }
+    const int32_t n0 = A.n, t0 = A.ntiles;
+    const int64_t s0 = A.s;
+    const uint32_t c0 = (uint32_t)A.lifc.size();
+    const int64_t a0 = (int64_t)A.arr_t.size();
+    const uint32_t p0 = (uint32_t)A.pois_tab.size();
+    const int64_t r0 = (int64_t)A.rng.size();
+    const int64_t o0 = (int64_t)A.off_tab.size();
+    const int32_t u0 = (int32_t)A.src_lo.size();
+    const int64_t l0 = A.src_list_len;
+    const int64_t k0 = (int64_t)A.chunk_tiles.size();
+    if (a0 + (int64_t)B.arr_t.size() > 0x7FFFFFFF || (uint64_t)p0 + B.pois_tab.size() > 0xFFFFFFFFull ||
+        (uint64_t)c0 + B.lifc.size() > BSIM_CLS_SET_MASK) {
+        err = "append_layout: source tables too large";
+        return -1;
+    }
+    for (int32_t i = 0; i < B.n; i++) {
+        const uint32_t kind = B.ncls[i] >> 30;
+        uint32_t cls = B.ncls[i];
+        int32_t cur = B.arr_cur0[i], end = B.arr_end[i];
+        uint32_t bp = B.bern_p[i];
+        if (kind == BSIM_KIND_LIF) {
+            cls += c0;
+        } else if (kind == BSIM_KIND_ARRAY) {
+            cur += (int32_t)a0;
+            end += (int32_t)a0;
+        } else if (kind == BSIM_KIND_POISSON) {
+            bp += p0;   /* arr_cur0 / arr_end of a Poisson source are cycles */
+        }
+        A.ncls.push_back(cls);
+        A.arr_cur0.push_back(cur);
+        A.arr_end.push_back(end);
+        A.bern_p.push_back(bp);
+    }
+    A.lifc.insert(A.lifc.end(), B.lifc.begin(), B.lifc.end());
+    A.v_tmp.insert(A.v_tmp.end(), B.v_tmp.begin(), B.v_tmp.end());
+    A.v_init.insert(A.v_init.end(), B.v_init.begin(), B.v_init.end());
+    A.arr_t.insert(A.arr_t.end(), B.arr_t.begin(), B.arr_t.end());
+    A.bern_seed.insert(A.bern_seed.end(), B.bern_seed.begin(), B.bern_seed.end());
+    A.bern_key.insert(A.bern_key.end(), B.bern_key.begin(), B.bern_key.end());
+    A.pois_tab.insert(A.pois_tab.end(), B.pois_tab.begin(), B.pois_tab.end());
+    /* tiles */
+    A.tile_syn.pop_back();
+    for (size_t t = 0; t < B.tile_syn.size(); t++) A.tile_syn.push_back(B.tile_syn[t] + s0);
+    for (int32_t t = 0; t < B.ntiles; t++) {
+        A.tile_base.push_back(B.tile_base[t] + s0);
+        A.tile_nsyn.push_back(B.tile_nsyn[t]);
+        A.tile_res.push_back(B.tile_res[t]);
+    }
+    A.syn_tgt.insert(A.syn_tgt.end(), B.syn_tgt.begin(), B.syn_tgt.end());
+    A.syn_w.insert(A.syn_w.end(), B.syn_w.begin(), B.syn_w.end());
+    /* source ranges, row tables (group-relative offsets: copied), chunks */
+    A.tile_rng.pop_back();
+    for (size_t t = 0; t < B.tile_rng.size(); t++) A.tile_rng.push_back(B.tile_rng[t] + (int32_t)r0);
+    for (bsim_range r : B.rng) {
+        r.lo += n0;
+        r.hi += n0;
+        r.off_base += o0;
+        r.uid += u0;
+        A.rng.push_back(r);
+    }
+    A.off_tab.insert(A.off_tab.end(), B.off_tab.begin(), B.off_tab.end());
+    for (size_t u = 0; u < B.src_lo.size(); u++) {
+        A.src_lo.push_back(B.src_lo[u] + n0);
+        A.src_hi.push_back(B.src_hi[u] + n0);
+        A.src_list_base.push_back(B.src_list_base[u] + l0);
+    }
+    A.src_list_len += B.src_list_len;
+    A.chunk_tiles_off.pop_back();
+    for (size_t u = 0; u < B.chunk_tiles_off.size(); u++) A.chunk_tiles_off.push_back(B.chunk_tiles_off[u] + k0);
+    for (int32_t t : B.chunk_tiles) A.chunk_tiles.push_back(t + t0);
+    A.chunk_wrow.insert(A.chunk_wrow.end(), B.chunk_wrow.begin(), B.chunk_wrow.end());
+    A.n += B.n;
+    A.ntiles += B.ntiles;
+    A.s += B.s;
+    A.ngroups += B.ngroups;
+    std::vector<int64_t> rcount(8, 0);
+    for (uint8_t x : A.tile_res) rcount[x]++;
+    A.nslots = (int32_t)(8 * *std::max_element(rcount.begin(), rcount.end()));
+    B = Layout();
     return 0;
 }
 

@@ -94,8 +94,18 @@ int32_t default_tile(int32_t dmax);
 /* Largest round(delay / dt) over the synapses (at least 1). */
 int32_t max_delay_steps(const HostNet &net, float dt);
 
-/* Returns 0 or a negative code with a message in err. */
-int build_layout(const HostNet &net, float dt, int32_t tile, Layout &out, std::string &err);
+/* Returns 0 or a negative code with a message in err.  The delay ring is at
+ * least dmax_min steps deep (a part of a larger layout uses the whole
+ * network's depth). */
+int build_layout(const HostNet &net, float dt, int32_t tile, Layout &out, std::string &err,
+                 int32_t dmax_min = 0);
+
+/* Appends layout B -- built, with the same tile and delay ring depth, for a
+ * network placed after A's neurons, with no synapse between the two -- to A:
+ * B's neuron ids, tiles, synapse offsets, source ranges, row-table offsets,
+ * chunks and tables are shifted past A's, which must end on a tile
+ * boundary.  B is emptied.  Returns 0 or a negative code with err set. */
+int append_layout(Layout &A, Layout &B, std::string &err);
 
 /* Appends the inverse-CDF table of Poisson(rate) to tab (format in layout.h). */
 void poisson_table(float rate, std::vector<uint32_t> &tab);

@@ -290,6 +290,42 @@ struct Acc {
     float e[S_], i[S_];
 };
 
+/* The state of a streaming wavefront's neurons that phase (i) reads, loaded
+ * before phase (ii) so that its latency is hidden behind the delivery (no
+ * other workgroup writes these words during the launch: a neuron belongs to
+ * one tile, and the history words read are the previous cycle's copy).  Kept
+ * in registers up to PREFETCH_MAX_S slots per lane. */
+#ifndef PREFETCH_MAX_S
+#define PREFETCH_MAX_S 1
+#endif
+template <int S_>
+struct NState {
+    uint32_t cls[S_];
+    int32_t refrac[S_];
+    float v[S_], ie[S_], ii[S_], vtmp[S_];
+    uint64_t hist[S_];
+};
+
+template <int S_>
+__device__ __forceinline__ void load_nstate(const bsim_dev &d, int32_t base, const uint64_t *hist_in,
+                                            NState<S_> &ns)
+{
+    const int32_t lane = (int32_t)(threadIdx.x & (WAVE - 1));
+#pragma unroll
+    for (int j = 0; j < S_; j++) {
+        const int32_t nid = base + j * WAVE + lane;
+        const bool valid = nid < d.n;
+        const int32_t x = valid ? nid : 0;
+        ns.cls[j] = d.ncls[x];
+        ns.refrac[j] = d.refrac[x];
+        ns.v[j] = d.v[x];
+        ns.ie[j] = d.ie[x];
+        ns.ii[j] = d.ii[x];
+        ns.vtmp[j] = d.v_tmp[x];
+        ns.hist[j] = hist_in[x];
+    }
+}
+
 /* Adds one owned event to the accumulator of its slot (E if w >= 0, else I)
  * when `valid`.  Written as selects over statically indexed accumulators, so
  * that they stay in registers for every S_ and no branch separates the staged
@@ -570,6 +606,13 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
         if (lane < NGRP) S.pmask[w * NGRP + lane] = 0ull;
     }
 
+    constexpr bool PRE = S_ <= PREFETCH_MAX_S;
+    NState<PRE ? S_ : 1> ns;
+    if constexpr (PRE) {
+        if ((flags & 2) && w < NWAVE && k >= 0)
+            load_nstate<S_>(d, k * T + (int32_t)qbase, d.hist + (int64_t)((t - 1) & 1) * d.n, ns);
+    }
+
     /* ---- phase (ii) of cycle t-1 ---------------------------------------- */
     if ((flags & 1) && k >= 0) {
         const int64_t syn0 = d.tile_base[k];
@@ -600,18 +643,26 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
             bool fired = false;
             uint64_t h = 0;
             if (valid) {
-                const uint32_t cls = d.ncls[nid];
+                uint32_t cls;
+                int32_t rs;
+                float v, ie, ii, vtmp;
+                uint64_t hin;
+                if constexpr (PRE) {
+                    cls = ns.cls[j]; rs = ns.refrac[j]; v = ns.v[j]; ie = ns.ie[j]; ii = ns.ii[j];
+                    vtmp = ns.vtmp[j]; hin = ns.hist[j];
+                } else {
+                    cls = d.ncls[nid]; rs = d.refrac[nid]; v = d.v[nid]; ie = d.ie[nid]; ii = d.ii[nid];
+                    vtmp = d.v_tmp[nid]; hin = hist_in[nid];
+                }
                 const uint32_t kind = cls >> 30;
                 if (kind == BSIM_KIND_LIF) {
                     const float inE = acc.e[j];
                     const float inI = acc.i[j];
-                    const int32_t rs = d.refrac[nid];
                     if (rs > 0) {
                         d.refrac[nid] = rs - 1;
                     } else {
                         const bsim_lifc c = d.lifc[cls & BSIM_CLS_SET_MASK];
-                        float v = d.v[nid], ie = d.ie[nid], ii = d.ii[nid];
-                        v = c.Cm * v + d.v_tmp[nid] + ie * c.C_E + ii * c.C_I;
+                        v = c.Cm * v + vtmp + ie * c.C_E + ii * c.C_I;
                         ie = c.CE * ie;
                         ii = c.CI * ii;
                         if (v >= c.v_thresh) {
@@ -647,7 +698,7 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
                 } else {
                     fired = bern_hash(d.bern_seed[nid], d.bern_key[nid], (uint32_t)t) < d.bern_p[nid];
                 }
-                h = (hist_in[nid] << 1) | (fired ? 1ull : 0ull);
+                h = (hin << 1) | (fired ? 1ull : 0ull);
                 hist_out[nid] = h;
                 if (d.slot_count) d.slot_count[nid] += __popcll((h >> 1) & dmask);
                 if (fired) d.fire_count[nid] += 1;
@@ -655,10 +706,10 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
                     const int32_t ri = d.rec_index[nid];
                     if (ri >= 0) {
                         const int64_t o = (int64_t)(t - d.rec_t0) * d.n_rec + ri;
-                        d.rec_v[o] = d.v[nid];
+                        d.rec_v[o] = kind == BSIM_KIND_LIF ? v : 0.0f;
                         if (d.rec_ie) {
-                            d.rec_ie[o] = d.ie[nid];
-                            d.rec_ii[o] = d.ii[nid];
+                            d.rec_ie[o] = kind == BSIM_KIND_LIF ? ie : 0.0f;
+                            d.rec_ii[o] = kind == BSIM_KIND_LIF ? ii : 0.0f;
                             d.rec_in_e[o] = kind == BSIM_KIND_LIF ? acc.e[j] : 0.0f;
                             d.rec_in_i[o] = kind == BSIM_KIND_LIF ? acc.i[j] : 0.0f;
                         }

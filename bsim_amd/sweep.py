@@ -179,6 +179,30 @@ def run_batch(points, steps, tile=0, record=True):
     return rasters, secs, dict(info, events=ev)
 
 
+def run_batch_parts(points, steps, tile=0, record=True, part_synapses=int(1.5e9)):
+    """run_batch for sweeps too large for one host build: the networks are
+    generated, laid out and uploaded in parts of at most part_synapses
+    synapses and joined on the device into one simulation
+    (bsim_sim_create_chains).  Returns (per-network rasters or None, seconds
+    of the stepping, simulation info)."""
+    sim, first = api.Simulation.from_chains([p.kwargs() for p in points], netspec.DT, tile=tile,
+                                            rec_steps=steps if record else 0,
+                                            part_synapses=part_synapses)
+    t0 = time.perf_counter()
+    sim.step(steps)
+    sim.finish()
+    sim.sync()
+    secs = time.perf_counter() - t0
+    info = sim.info()
+    rasters = None
+    if record:
+        counts, ids, _ = sim.raster()
+        rasters = split_raster(counts, ids, first, np.array([p.n for p in points], np.int64))
+    ev = sim.events()
+    sim.close()
+    return rasters, secs, dict(info, events=ev)
+
+
 def run_solo(point, steps, tile=0, record=True):
     """Simulates one point alone; returns (raster or None, seconds of stepping)."""
     net = api.Network()

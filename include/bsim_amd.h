@@ -265,6 +265,20 @@ typedef struct bsim_chain_params {
     float dt;
 } bsim_chain_params;
 
+/* One simulation of count sweep networks built without holding them all on
+ * the host: the networks are generated, laid out and uploaded in parts of at
+ * most part_synapses synapses (at least one network each), the parts joined
+ * on the device into one layout -- each part padded with inert neurons to a
+ * whole number of tiles -- and stepped as one network by bsim_sim_step.  No
+ * counterpart in the reference (its single-GPU simulator runs one network);
+ * it serves parameter sweeps whose networks do not fit one host build.
+ * first_ids[c] receives network c's first neuron id (the padding shifts
+ * them from what bsim_net_gen_chains would give).  cfg as for
+ * bsim_sim_create; recorded ids refer to the joined network. */
+bsim_sim *bsim_sim_create_chains(const bsim_chain_params *p, int32_t count, float dt,
+                                 const bsim_sim_config *cfg, int64_t part_synapses,
+                                 int32_t *first_ids);
+
 /* Appends one sweep network to net (which may hold others: a batch) and
  * returns the id of its first neuron.  Its draws, Bernoulli keys included,
  * depend only on p, so the network behaves in a batch exactly as alone. */

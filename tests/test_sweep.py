@@ -139,3 +139,23 @@ def test_full_size_sweep_batch_equals_solo():
     o = oracle.run(net.export_spec(), DT, steps, mode="csr", tile=2048, threads=8)
     np.testing.assert_array_equal(np.array([len(x) for x in rasters[k]]), o.raster_count)
     np.testing.assert_array_equal(np.concatenate(rasters[k]), o.raster_ids)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("tile", [0, 192])
+def test_batch_built_in_parts_equals_one_build(tile):
+    """A batch laid out in parts joined on the device (bsim_sim_create_chains,
+    here forced into several parts, each padded to whole tiles) gives every
+    network the raster of the batch built in one host network, and the same
+    deliveries."""
+    steps = 400
+    pts = sweep.sample(12, seed=19, **SMALL)
+    one, _, info1 = sweep.run_batch(pts, steps, tile=tile)
+    per = max(p.fan_in * sum(p.sizes[1:]) + p.drive_fan_in * p.sizes[0] for p in pts)
+    parts, _, info2 = sweep.run_batch_parts(pts, steps, tile=tile, part_synapses=2 * per)
+    assert info2["n"] >= info1["n"]           # inert padding between the parts
+    assert info2["s"] == info1["s"] and info2["events"] == info1["events"]
+    for k in range(len(pts)):
+        for c in range(steps):
+            np.testing.assert_array_equal(parts[k][c], one[k][c], err_msg=f"network {k} cycle {c}")
+    assert sum(len(x) for r in one for x in r) > 0
