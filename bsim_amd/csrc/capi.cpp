@@ -505,6 +505,7 @@ static bsim_sim *sim_from_layout(bsim::Layout &L, const bsim_sim_config &c, bsim
     UP(src_list_base, L.src_list_base);
     UP(chunk_tiles_off, L.chunk_tiles_off);
     UP(chunk_tiles, L.chunk_tiles);
+    UP(chunk_rng, L.chunk_rng);
     UP(chunk_wrow, L.chunk_wrow);
     d.nsrc = (int32_t)L.src_lo.size();
     /* state */
@@ -539,9 +540,9 @@ static bsim_sim *sim_from_layout(bsim::Layout &L, const bsim_sim_config &c, bsim
         if (!(d.tile_load = dev_alloc<uint64_t>(sim, nt, e))) goto hip_fail;
         if ((e = hipMemsetAsync(d.tile_load, 0, nt * sizeof(uint64_t), sim->stream)) != hipSuccess) goto hip_fail;
         if (!(d.tile_cls = dev_alloc<uint8_t>(sim, nt, e))) goto hip_fail;
-        const size_t ns = (size_t)(d.nsrc > 0 ? d.nsrc : 1);
-        if (!(d.src_count = dev_alloc<uint32_t>(sim, ns, e))) goto hip_fail;
-        if ((e = hipMemsetAsync(d.src_count, 0, ns * sizeof(uint32_t), sim->stream)) != hipSuccess) goto hip_fail;
+        const size_t nre = (size_t)(L.rng.empty() ? 1 : L.rng.size());
+        if (!(d.rng_count = dev_alloc<uint32_t>(sim, nre, e))) goto hip_fail;
+        if ((e = hipMemsetAsync(d.rng_count, 0, nre * sizeof(uint32_t), sim->stream)) != hipSuccess) goto hip_fail;
         if (!(d.src_list = dev_alloc<uint32_t>(sim, (size_t)(L.src_list_len > 0 ? L.src_list_len : 1), e)))
             goto hip_fail;
         if ((e = hipMemsetAsync(d.ie, 0, n * 4, sim->stream)) != hipSuccess) goto hip_fail;

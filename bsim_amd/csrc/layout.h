@@ -60,6 +60,7 @@ enum { BSIM_KIND_LIF = 0, BSIM_KIND_ARRAY = 1, BSIM_KIND_BERNOULLI = 2, BSIM_KIN
 typedef struct bsim_range {
     int32_t lo, hi;     /* pre neurons [lo, hi) of one chunk of a source range of the tile */
     int64_t off_base;   /* off_tab index of the tile's vector of row (lo, 1) */
+    int64_t lbase;      /* first entry of the chunk's active-row list in src_list */
     int32_t uid;        /* source chunk: index into src_* */
     uint32_t wrow;      /* expected deliveries of one active row into the tile, x 16 */
     uint32_t stride;    /* off_tab entries per (pre, delay) row of the group */
@@ -137,7 +138,7 @@ typedef struct bsim_dev {
     const int32_t *src_lo, *src_hi;
     const int64_t *src_list_base;
     uint32_t *src_list;
-    uint32_t *src_count;
+    uint32_t *rng_count;        /* [range entries] rows in the entry's chunk list this cycle */
     /* accounting / recording */
     int64_t *ev_count;          /* [ntiles] deliveries performed */
     int32_t *clock;             /* [4]: cycle of the next launch, step workgroups finished,
@@ -147,6 +148,7 @@ typedef struct bsim_dev {
     uint8_t *tile_cls;          /* [ntiles] its half-octave class */
     const int64_t *chunk_tiles_off;  /* [nsrc + 1] tiles that read each chunk: */
     const int32_t *chunk_tiles;      /*   tile ids */
+    const int32_t *chunk_rng;        /*   the tile's range entry for the chunk */
     const uint32_t *chunk_wrow;      /*   expected deliveries per active row, x 16 */
     uint64_t *raster;           /* [rec_steps][nwords] fired bitmaps, or NULL */
     int32_t rec_steps;

@@ -572,6 +572,7 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
                     rc.off_base = gtab_off[g] + gtab_base[g][ri] +
                                   (int64_t)(c - r.lo) * D * rc.stride + (int64_t)(t - k0) * RW;
                     rc.uid = u;
+                    rc.lbase = L.src_list_base[(size_t)u];
                     rc.pad = 0;
                     /* synapses of the tile from the chunk, per (pre, delay) row */
                     const size_t e0 = (size_t)(r.off_base + (int64_t)(c - r.lo) * D * NW);
@@ -595,6 +596,7 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
     for (const auto &r : L.rng) L.chunk_tiles_off[(size_t)r.uid + 1]++;
     for (size_t u = 0; u < nch; u++) L.chunk_tiles_off[u + 1] += L.chunk_tiles_off[u];
     L.chunk_tiles.resize(L.rng.size());
+    L.chunk_rng.resize(L.rng.size());
     L.chunk_wrow.resize(L.rng.size());
     {
         std::vector<int64_t> fill(L.chunk_tiles_off.begin(), L.chunk_tiles_off.end() - 1);
@@ -602,6 +604,7 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
             for (int32_t ri = L.tile_rng[t]; ri < L.tile_rng[t + 1]; ri++) {
                 const int64_t x = fill[L.rng[ri].uid]++;
                 L.chunk_tiles[x] = t;
+                L.chunk_rng[x] = ri;
                 L.chunk_wrow[x] = L.rng[ri].wrow;
             }
     }
@@ -689,6 +692,7 @@ int append_layout(Layout &A, Layout &B, std::string &err)
         r.hi += n0;
         r.off_base += o0;
         r.uid += u0;
+        r.lbase += l0;
         A.rng.push_back(r);
     }
     A.off_tab.insert(A.off_tab.end(), B.off_tab.begin(), B.off_tab.end());
@@ -701,6 +705,7 @@ int append_layout(Layout &A, Layout &B, std::string &err)
     A.chunk_tiles_off.pop_back();
     for (size_t u = 0; u < B.chunk_tiles_off.size(); u++) A.chunk_tiles_off.push_back(B.chunk_tiles_off[u] + k0);
     for (int32_t t : B.chunk_tiles) A.chunk_tiles.push_back(t + t0);
+    for (int32_t x : B.chunk_rng) A.chunk_rng.push_back(x + (int32_t)r0);
     A.chunk_wrow.insert(A.chunk_wrow.end(), B.chunk_wrow.begin(), B.chunk_wrow.end());
     A.n += B.n;
     A.ntiles += B.ntiles;

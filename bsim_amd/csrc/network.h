@@ -76,6 +76,7 @@ struct Layout {
     std::vector<uint8_t> tile_res;           /* per tile: XCD residue of its group */
     std::vector<int64_t> chunk_tiles_off;    /* per chunk: the tiles reading it */
     std::vector<int32_t> chunk_tiles;
+    std::vector<int32_t> chunk_rng;          /* range entry of each (chunk, tile) */
     std::vector<uint32_t> chunk_wrow;
 };
 

@@ -64,6 +64,9 @@
 #define MAXR 128                      /* source ranges per batch */
 #define ROWS_THREADS 1024             /* row-list kernel: 4 pres per thread */
 #define HEAVY_WGS 256                 /* workgroups run at raised wave priority */
+#ifndef RANK_EVERY
+#define RANK_EVERY 1                  /* cycles between tile rankings */
+#endif
 static_assert(BSIM_CHUNK == 4 * ROWS_THREADS, "a source chunk is one row-list workgroup");
 
 #ifndef MARK_MIN_S
@@ -511,8 +514,8 @@ __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, in
             c[x] = 0u;
             if (i < nrb) {
                 const bsim_range r = d.rng[r0 + i];
-                c[x] = d.src_count[r.uid];
-                S.lbase[i] = d.src_list_base[r.uid];
+                c[x] = d.rng_count[r0 + i];
+                S.lbase[i] = r.lbase;
                 S.obase[i] = r.off_base;
                 S.stride[i] = r.stride;
             }
@@ -880,12 +883,21 @@ __global__ __launch_bounds__(ROWS_THREADS) void bsim_rows_kernel(bsim_dev d)
         const uint32_t rb = (uint32_t)(p0 + x - lo) * (uint32_t)D;
         for (uint64_t mm = m[x]; mm; mm &= mm - 1) out[pos++] = rb + (uint32_t)__builtin_ctzll(mm);
     }
-    if (tid == 0) d.src_count[u] = tot;
-    /* expected work of the tiles that read this chunk: rows weighted by the
-     * tile's mean deliveries per row plus a fixed cost of two */
-    for (int64_t x = d.chunk_tiles_off[u] + tid; x < d.chunk_tiles_off[u + 1]; x += ROWS_THREADS)
-        atomicAdd((unsigned long long *)&d.tile_load[d.chunk_tiles[x]],
-                  (unsigned long long)tot * (unsigned long long)(d.chunk_wrow[x] + 32u));
+    /* the tiles are ranked every RANK_EVERY cycles (the order only affects
+     * speed); d.clock[0], the next step launch's cycle, is the same for every
+     * workgroup of this launch */
+    const bool rank = RANK_EVERY == 1 || d.clock[0] % RANK_EVERY == 0;
+    /* for every tile that reads this chunk: the list's length in the tile's
+     * range entry and, when ranking, the tile's expected work -- rows
+     * weighted by the tile's mean deliveries per row plus a fixed cost of
+     * two */
+    for (int64_t x = d.chunk_tiles_off[u] + tid; x < d.chunk_tiles_off[u + 1]; x += ROWS_THREADS) {
+        d.rng_count[d.chunk_rng[x]] = tot;
+        if (rank)
+            atomicAdd((unsigned long long *)&d.tile_load[d.chunk_tiles[x]],
+                      (unsigned long long)tot * (unsigned long long)(d.chunk_wrow[x] + 32u));
+    }
+    if (!rank) return;
 
     /* the last row-list workgroup to finish ranks the tiles for the coming
      * step launch.  No fence: the ranking only orders the tiles, and any
