@@ -35,6 +35,7 @@
 #include <cmath>
 #include <cstdint>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -758,21 +759,26 @@ int bsim_mlp_set_data(bsim_mlp *m, const float *x, const float *y, int64_t n)
     return rc;
 }
 
-int bsim_mlp_train(bsim_mlp *m, const int32_t *rows, int32_t steps, int32_t batch, float lr,
-                   float *loss)
+}  // extern "C"
+
+namespace {
+
+int check_train(bsim_mlp *m, const int32_t *rows, int32_t steps, int32_t batch)
 {
     if (!m || !rows || steps < 0) return mfail(BSIM_E_ARG, "null argument");
     if (int rc = check_batch(m, batch)) return rc;
     if (m->ndata <= 0) return mfail(BSIM_E_ARG, "no data set (bsim_mlp_set_data)");
     for (int64_t i = 0; i < (int64_t)steps * batch; i++)
         if (rows[i] < 0 || rows[i] >= m->ndata) return mfail(BSIM_E_ARG, "row index out of range");
-    if (steps == 0) return BSIM_OK;
-    float *ldev = nullptr;
-    int32_t *rdev = nullptr;
-    int rc = dalloc(m, &ldev, steps);
-    if (!rc) rc = dalloc(m, &rdev, (int64_t)steps * batch);
-    if (!rc) rc = hip_check(hipMemcpy(rdev, rows, (size_t)steps * batch * 4, hipMemcpyHostToDevice),
-                            "hipMemcpy");
+    return BSIM_OK;
+}
+
+/* The training steps of bsim_mlp_train and bsim_mlp_train_noalloc: the
+ * minibatch row indices are in rdev (device), step s's loss goes to ldev[s]
+ * (device); synchronises the stream. */
+int train_steps(bsim_mlp *m, const int32_t *rdev, float *ldev, int32_t steps, int32_t batch, float lr)
+{
+    int rc = BSIM_OK;
     for (int32_t s = 0; s < steps && !rc; s++) {
         hipLaunchKernelGGL(mlp_gather, dim3(batch), dim3(256), 0, m->stream, m->xall, m->yall,
                            rdev + (int64_t)s * batch, m->act[0], m->act_t[0], m->ybat, batch,
@@ -784,13 +790,84 @@ int bsim_mlp_train(bsim_mlp *m, const int32_t *rows, int32_t steps, int32_t batc
         if (!rc) rc = transpose_weights(m);
     }
     const int src = hip_check(hipStreamSynchronize(m->stream), "sync");
-    if (!rc) rc = src;
+    return rc ? rc : src;
+}
+
+/* Scratch of bsim_mlp_train_noalloc: the per-step losses (from word 0), then
+ * the minibatch row indices (from word round_up(steps, 64)).  A file-scope
+ * device array, one copy per GPU, held until the process ends; its address
+ * on each device is taken once, with that device current, by
+ * hipGetSymbolAddress (the array's name is only a host-side stand-in). */
+static __device__ __align__(256) uint32_t scratch[BSIM_MLP_NOALLOC_WORDS];
+
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mu;
+uint32_t *g_scratch_ptr[kMaxDevices] = {};
+
+int scratch_on(int device, uint32_t **out)
+{
+    if (device < 0 || device >= kMaxDevices) return mfail(BSIM_E_ARG, "device ordinal out of range");
+    std::lock_guard<std::mutex> lock(g_scratch_mu);
+    if (!g_scratch_ptr[device]) {
+        void *ptr = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch));
+        if (e != hipSuccess) return hip_check(e, "hipGetSymbolAddress(scratch)");
+        g_scratch_ptr[device] = static_cast<uint32_t *>(ptr);
+    }
+    *out = g_scratch_ptr[device];
+    return BSIM_OK;
+}
+
+}  // namespace
+
+extern "C" {
+
+int bsim_mlp_train(bsim_mlp *m, const int32_t *rows, int32_t steps, int32_t batch, float lr,
+                   float *loss)
+{
+    if (int rc = check_train(m, rows, steps, batch)) return rc;
+    if (steps == 0) return BSIM_OK;
+    float *ldev = nullptr;
+    int32_t *rdev = nullptr;
+    int rc = dalloc(m, &ldev, steps);
+    if (!rc) rc = dalloc(m, &rdev, (int64_t)steps * batch);
+    if (!rc) rc = hip_check(hipMemcpy(rdev, rows, (size_t)steps * batch * 4, hipMemcpyHostToDevice),
+                            "hipMemcpy");
+    if (!rc) rc = train_steps(m, rdev, ldev, steps, batch, lr);
     if (!rc && loss)
         rc = hip_check(hipMemcpy(loss, ldev, steps * 4, hipMemcpyDeviceToHost), "hipMemcpy");
     for (void *q : {(void *)ldev, (void *)rdev}) {
         if (!q) continue;
         m->allocs.erase(std::find(m->allocs.begin(), m->allocs.end(), q));
         hipFree(q);
+    }
+    return rc;
+}
+
+int bsim_mlp_train_noalloc(bsim_mlp *m, const int32_t *rows, int32_t steps, int32_t batch,
+                           float lr, float *loss)
+{
+    if (int rc = check_train(m, rows, steps, batch)) return rc;
+    const int64_t roff = round_up(steps, 64);
+    if (roff + (int64_t)steps * batch > (int64_t)BSIM_MLP_NOALLOC_WORDS)
+        return mfail(BSIM_E_ARG, "bsim_mlp_train_noalloc: round_up(steps, 64) + steps * batch exceeds "
+                                 "the scratch of " + std::to_string(BSIM_MLP_NOALLOC_WORDS) + " words");
+    if (steps == 0) return BSIM_OK;
+    int rc = hip_check(hipSetDevice(m->cfg.device), "hipSetDevice");
+    uint32_t *base = nullptr;
+    if (!rc) rc = scratch_on(m->cfg.device, &base);
+    if (rc) return rc;
+    float *ldev = reinterpret_cast<float *>(base);
+    int32_t *rdev = reinterpret_cast<int32_t *>(base + roff);
+    /* every word the steps read is written here first; the losses are
+     * stored, not accumulated: nothing needs clearing */
+    rc = hip_check(hipMemcpyAsync(rdev, rows, (size_t)steps * batch * 4, hipMemcpyHostToDevice,
+                                  m->stream), "hipMemcpyAsync");
+    if (!rc) rc = train_steps(m, rdev, ldev, steps, batch, lr);
+    if (!rc && loss) {
+        rc = hip_check(hipMemcpyAsync(loss, ldev, (size_t)steps * 4, hipMemcpyDeviceToHost, m->stream),
+                       "hipMemcpyAsync");
+        if (!rc) rc = hip_check(hipStreamSynchronize(m->stream), "sync");
     }
     return rc;
 }

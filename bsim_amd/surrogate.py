@@ -36,6 +36,7 @@ _SIGS = {
     "bsim_mlp_set_params": (C.c_int32, [P, P]),
     "bsim_mlp_set_data": (C.c_int32, [P, P, P, C.c_int64]),
     "bsim_mlp_train": (C.c_int32, [P, P, C.c_int32, C.c_int32, C.c_float, P]),
+    "bsim_mlp_train_noalloc": (C.c_int32, [P, P, C.c_int32, C.c_int32, C.c_float, P]),
     "bsim_mlp_gradients": (C.c_int32, [P, P, P, C.c_int32, P, P]),
     "bsim_mlp_predict": (C.c_int32, [P, P, C.c_int64, P]),
 }
@@ -119,6 +120,17 @@ class MLP:
         loss = np.zeros(max(steps, 1), np.float32)
         _check(lib().bsim_mlp_train(self._h, rows.ctypes.data, steps, batch, float(lr),
                                     loss.ctypes.data), "bsim_mlp_train")
+        return loss[:steps]
+
+    def train_noalloc(self, rows, batch, lr):
+        """train() through bsim_mlp_train_noalloc: no device allocation per
+        call (fixed per-GPU scratch); round_up(steps, 64) + steps * batch must
+        fit BSIM_MLP_NOALLOC_WORDS."""
+        rows = np.ascontiguousarray(np.asarray(rows, np.int32))
+        steps = rows.shape[0]
+        loss = np.zeros(max(steps, 1), np.float32)
+        _check(lib().bsim_mlp_train_noalloc(self._h, rows.ctypes.data, steps, batch, float(lr),
+                                            loss.ctypes.data), "bsim_mlp_train_noalloc")
         return loss[:steps]
 
     def gradients(self, x, y):

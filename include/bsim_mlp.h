@@ -67,6 +67,20 @@ int bsim_mlp_set_data(bsim_mlp *m, const float *x, const float *y, int64_t n);
  * loss is not NULL.  batch: a multiple of 256, at most max_batch. */
 int bsim_mlp_train(bsim_mlp *m, const int32_t *rows, int32_t steps, int32_t batch, float lr,
                    float *loss);
+/* bsim_mlp_train for callers that cannot allocate device memory (code under
+ * HIP graph capture, tight training or serving loops, harnesses that forbid
+ * hipMalloc).  No counterpart in the reference.  Same arguments, kernels and
+ * results as bsim_mlp_train, but the per-call staging (the minibatch row
+ * indices and the per-step losses) lives in a fixed device array of
+ * BSIM_MLP_NOALLOC_WORDS 32-bit words per GPU, allocated with the library
+ * and held until the process ends, instead of a hipMalloc per call:
+ * round_up(steps, 64) + steps * batch must not exceed it (255 steps at batch
+ * 4,096), or BSIM_E_ARG is returned.  Calls on one GPU share the array:
+ * calls that overlap on one GPU (from several threads or handles) race. */
+#define BSIM_MLP_NOALLOC_WORDS (1 << 20)
+int bsim_mlp_train_noalloc(bsim_mlp *m, const int32_t *rows, int32_t steps, int32_t batch,
+                           float lr, float *loss);
+
 /* Gradient of the loss for one batch x[batch][in_dim], y[batch][out_dim],
  * in the flat parameter layout, without updating anything. */
 int bsim_mlp_gradients(bsim_mlp *m, const float *x, const float *y, int32_t batch, float *grad,
