@@ -255,3 +255,25 @@ def test_standard_topology_program_matches_reference(tmp_path, name):
     gv = np.loadtxt(tmp_path / "g_v.data", ndmin=2)
     np.testing.assert_allclose(gv, gold["rec_v"].astype(np.float64), rtol=0, atol=5.1e-11)
     assert len(ids) > 1000
+
+
+def test_truncated_input_fails_cleanly(tmp_path):
+    """A truncated network file makes the program stop with a message and a
+    non-zero status before any device work (the reference prints and
+    returns -1 on file errors)."""
+    if not os.path.exists(BIN):
+        pytest.skip("tests/cpp/dropin_run is not built")
+    meta, spec, dtype, gold = golden_io.load("c0_f32")
+    n = 1000
+    inp = tmp_path / "c0.bin"
+    write_input(inp, spec, [(0, n), (n, n)], [(0, 1, spec.w, spec.d)], np.float32(1e-4),
+                np.float32(0.01), [0])
+    data = inp.read_bytes()
+    for cut in (3, len(data) // 2, len(data) - 5):
+        bad = tmp_path / f"cut{cut}.bin"
+        bad.write_bytes(data[:cut])
+        r = subprocess.run([BIN, str(bad), str(tmp_path)], capture_output=True, text=True, timeout=60)
+        assert r.returncode != 0 and "short read" in r.stderr, (cut, r.returncode, r.stderr)
+    r = subprocess.run([BIN, str(tmp_path / "missing.bin"), str(tmp_path)], capture_output=True,
+                       text=True, timeout=60)
+    assert r.returncode != 0

@@ -224,3 +224,21 @@ def test_run_cycles_rounds_halves_away_from_zero():
     # the case numpy's round-half-to-even gets wrong: 4.5 steps -> 5 cycles
     assert api.run_cycles(np.float32(4.5) * dt / np.float32(1.0), dt) == \
         int(libm.roundf(float(np.float32(np.float32(4.5) * dt) / dt)))
+
+
+def test_host_layout_check_under_sanitizers():
+    """tests/cpp/host_layout_check.cpp, built from the library's host sources
+    under AddressSanitizer and UBSan: networks laid out in parts and joined
+    equal the same network laid out at once, array for array, and arbitrary
+    creation orders lay out without a memory error."""
+    import shutil
+    import subprocess
+    cpp = os.path.join(ROOT, "tests", "cpp")
+    if shutil.which("g++") is None:
+        pytest.skip("no g++")
+    r = subprocess.run(["make", "-s", "-C", cpp, "host_layout_check"], capture_output=True, text=True)
+    assert r.returncode == 0, r.stdout + r.stderr
+    env = dict(os.environ, OMP_NUM_THREADS="4", ASAN_OPTIONS="detect_leaks=1")
+    r = subprocess.run([os.path.join(cpp, "host_layout_check")], capture_output=True, text=True,
+                       env=env, timeout=600)
+    assert r.returncode == 0 and "host_layout_check ok" in r.stdout, r.stdout + r.stderr
