@@ -232,6 +232,7 @@ def bench_sweep(a):
     # timed without recording; then the sampled networks again with the
     # raster for the check
     t0 = time.time()
+    os.environ.setdefault("BSIM_PROGRESS", "1")     # one line per part of the build
     _, secs_b, info = sweep.run_batch_parts(pts, steps, tile=tile, record=False,
                                             part_synapses=int(a.c3_batch_syn))
     events, ntiles = info["events"], info["ntiles"]
@@ -245,6 +246,8 @@ def bench_sweep(a):
         secs_s += dt_solo
         solo, _ = sweep.run_solo(pts[i], steps, tile=tile, record=True)
         exact += int(all(np.array_equal(rast[j][c], solo[c]) for c in range(steps)))
+        print(f"[bench] solo {j + 1}/{len(sample)}: network {i}, {dt_solo:.3f}s, bit-exact so far "
+              f"{exact}/{j + 1}", file=sys.stderr, flush=True)
     nps_b = len(pts) / secs_b
     nps_s = len(sample) / secs_s
     print(f"[bench] one at a time: {secs_s:.3f}s for {len(sample)} networks; bit-exact "

@@ -22,6 +22,8 @@
  */
 #include <math.h>
 #include <stdint.h>
+#include <stdio.h>
+#include <stdlib.h>
 
 #include <algorithm>
 #include <functional>
@@ -430,6 +432,10 @@ int build_chains_layout(const bsim_chain_params *p, int32_t count, float dt, int
                         const std::function<int(const Layout &, int64_t)> &upload)
 {
     out = Layout();
+    /* per-part progress on stderr when BSIM_PROGRESS is set (large sweeps
+     * take minutes to build) */
+    const char *pe = getenv("BSIM_PROGRESS");
+    const bool progress = pe && pe[0] && pe[0] != '0';
     int32_t c0 = 0;
     while (c0 < count) {
         /* the next part: chains while their synapses fit part_synapses (at
@@ -477,6 +483,9 @@ int build_chains_layout(const bsim_chain_params *p, int32_t count, float dt, int
         if (append_layout(out, pl, err) != 0) return -1;
         if (first_ids)
             for (int32_t c = c0; c < c1; c++) first_ids[c] = base + first[(size_t)(c - c0)];
+        if (progress)
+            fprintf(stderr, "[bsim] chains %d..%d of %d laid out and uploaded: %d neurons, %lld synapses so far\n",
+                    c0, c1 - 1, count, out.n, (long long)out.s);
         c0 = c1;
     }
     return 0;
