@@ -64,9 +64,6 @@
 #define MAXR 128                      /* source ranges per batch */
 #define ROWS_THREADS 1024             /* row-list kernel: 4 pres per thread */
 #define HEAVY_WGS 256                 /* workgroups run at raised wave priority */
-#ifndef PRODUCER_PREFETCH
-#define PRODUCER_PREFETCH 0           /* the producer touches each staged row's first synapse */
-#endif
 #ifndef RANK_EVERY
 #define RANK_EVERY 4                  /* cycles between tile rankings */
 #endif
@@ -461,8 +458,7 @@ __device__ __forceinline__ void prod_entries(const Shared &S, const bsim_dev &d,
  * part.  The rows of a lane ascend, so their source range is found by walking
  * forward. */
 __device__ __forceinline__ void prod_stage(const Shared &S, const bsim_dev &d, int32_t buf, int32_t win,
-                                           uint32_t nrows_tot, int32_t nrb, const uint32_t (&E)[RPL],
-                                           const bsim_syn *syn, uint32_t nsyn, uint32_t &sink)
+                                           uint32_t nrows_tot, int32_t nrb, const uint32_t (&E)[RPL])
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     uint4 *rows = reinterpret_cast<uint4 *>(S.rows + buf * (RCAP * ROWW));
@@ -476,15 +472,6 @@ __device__ __forceinline__ void prod_stage(const Shared &S, const bsim_dev &d, i
             while (ri + 1 < nrb && S.cum[ri + 1] <= j) ri++;
             const int64_t e0 = S.obase[ri] + (int64_t)E[x] * S.stride[ri];
             Tb[x] = *reinterpret_cast<const uint4 *>(d.off_tab + e0);
-        }
-    }
-    if constexpr (PRODUCER_PREFETCH) {
-        /* touch the first synapse word of each staged row, so that the
-         * streaming wavefronts find the row's cache line in the L2 */
-#pragma unroll
-        for (int x = 0; x < RPL; x++) {
-            const uint32_t a = Tb[x].w > Tb[x].x ? Tb[x].x : 0u;
-            sink ^= syn[a < nsyn ? a : 0u].tgt;
         }
     }
     uint32_t ql[NWAVE];
@@ -551,10 +538,9 @@ __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, in
     const int32_t nwin = (int32_t)((nrows_tot + RCAP - 1) / RCAP);
     if (nwin == 0) return;                                    /* uniform */
     uint32_t E[RPL];
-    uint32_t sink = 0;
     if (prod) {
         prod_entries(S, d, 0, nrows_tot, nrb, E);
-        prod_stage(S, d, 0, 0, nrows_tot, nrb, E, syn, nsyn, sink);
+        prod_stage(S, d, 0, 0, nrows_tot, nrb, E);
         if (nwin > 1) prod_entries(S, d, 1, nrows_tot, nrb, E);
     }
     __syncthreads();
@@ -562,7 +548,7 @@ __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, in
         const int32_t buf = i & 1;
         if (prod) {
             if (i + 1 < nwin) {
-                prod_stage(S, d, buf ^ 1, i + 1, nrows_tot, nrb, E, syn, nsyn, sink);
+                prod_stage(S, d, buf ^ 1, i + 1, nrows_tot, nrb, E);
                 if (i + 2 < nwin) prod_entries(S, d, i + 2, nrows_tot, nrb, E);
             }
         } else {
@@ -572,12 +558,6 @@ __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, in
             stream_rows<S_>(S, S.rows + buf * (RCAP * ROWW), nrows, qtot, qbase, nsyn, syn, acc);
         }
         __syncthreads();
-    }
-    if constexpr (PRODUCER_PREFETCH) {
-        /* the touched words are consumed here, after the last window, so
-         * that their loads never hold up the staging */
-        __shared__ uint32_t sink_word;
-        if (prod && sink == 0x9E3779B9u) sink_word = sink;
     }
 }
 
