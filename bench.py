@@ -24,9 +24,11 @@ stream synchronisations.  One JSON line is printed:
                fp32 weight + int32 target, as the reference stores them) per
                launch over its mean launch duration, measured with HIP events
                recorded by the library around each launch on the simulation's
-               stream during a further 200-cycle pass; peak = 8 TB/s (MI355X
-               HBM3E).  The cycle's second kernel (active-row lists) is timed
-               the same way and reported beside it.
+               stream during --roofline-steps (200) single launches right
+               after the pre-roll -- the same cycles in every run, whatever K
+               and W; peak = 8 TB/s (MI355X HBM3E).  The cycle's second kernel
+               (active-row lists) is timed the same way and reported beside
+               it.  Then W warm-up cycles, then the K timed ones.
   cpu_baseline the oracle's event-driven CSR form (oracle/stsim.c) on the
                host's cores, timed over the same cycles of the same network
                that the GPU timed (the first --cpu-steps of them when K is
@@ -43,7 +45,9 @@ scaled by what the same counter reads for --copy-probe, a plain copy kernel of
 known size; --pmc-summary prints the per-dispatch averages of such a run.  The
 figures are read from profiles/step_kernel_<workload>_traffic.txt when that
 file exists for the same workload and tile and was measured at a load within
-10 % of this run's deliveries per launch (null otherwise).
+10 % of this run's deliveries per launch (null otherwise).  `--steps 0` runs
+the pre-roll and the roofline pass only (the counter runs: their last
+--roofline-steps step-kernel dispatches are the roofline cycles).
 """
 from __future__ import annotations
 
@@ -495,9 +499,28 @@ def main():
     ev_pre = sim.events()
     print(f"[bench] pre-rolled {a.preroll} cycles in {time.time() - t0:.2f}s ({ev_pre} events)",
           file=sys.stderr, flush=True)
+    # live kernel timing at a fixed point of the run (cycles preroll ..
+    # preroll + R, whatever K and W are): HIP events around each kernel of R
+    # single launches, recorded by the library on the simulation's stream
+    R = a.roofline_steps
+    evr0 = sim.events()
+    step_us, rows_us = sim.time_steps(R)
+    evr1 = sim.events()
+    mean_ms = step_us * 1e-3
+    ev_per_launch = (evr1 - evr0) / R
+    achieved = ev_per_launch * BYTES_PER_EVENT / (mean_ms * 1e-3) / 1e9
+    print(f"[bench] step kernel mean {step_us:.2f} us, row-list kernel mean {rows_us:.2f} us "
+          f"over {R} launches (cycles {a.preroll}..{a.preroll + R}), {ev_per_launch:.0f} "
+          f"events/launch", file=sys.stderr, flush=True)
+    traffic, traffic_ev = read_traffic(a.workload, info["tile"], ev_per_launch)
+    if a.steps == 0:            # a profiling run of the pre-roll and the roofline pass only
+        sim.close()
+        print(json.dumps({"roofline_only": True, "mean_launch_us": step_us,
+                          "events_per_launch": ev_per_launch}), flush=True)
+        return
     sim.step(a.warmup)
     sim.sync()
-    start = a.preroll + a.warmup
+    start = a.preroll + R + a.warmup
     # the first min(K, --cpu-steps) timed launches are the CPU baseline's
     # window; their deliveries are read at a stream synchronisation inside the
     # timed region (one extra host wait when K is larger).  Launch L (counted
@@ -527,19 +550,7 @@ def main():
           f"{events} events ({events / max(a.steps, 1):.4g} per cycle)", file=sys.stderr,
           flush=True)
 
-    # live kernel timing: HIP events around each kernel of a further pass,
-    # recorded by the library on the simulation's stream
-    R = a.roofline_steps
-    evr0 = sim.events()
-    step_us, rows_us = sim.time_steps(R)
-    evr1 = sim.events()
-    mean_ms = step_us * 1e-3
-    ev_per_launch = (evr1 - evr0) / R
-    achieved = ev_per_launch * BYTES_PER_EVENT / (mean_ms * 1e-3) / 1e9
-    print(f"[bench] step kernel mean {step_us:.2f} us, row-list kernel mean {rows_us:.2f} us "
-          f"over {R} launches, {ev_per_launch:.0f} events/launch", file=sys.stderr, flush=True)
     sim.close()
-    traffic, traffic_ev = read_traffic(a.workload, info["tile"], ev_per_launch)
 
     cpu = None
     cpu_ref = None
@@ -575,7 +586,8 @@ def main():
         "data": "synthetic (seeded generator)",
         "config": {"workload": desc, "name": a.workload, "neurons": n_neurons,
                    "synapses": n_syn, "dt_s": netspec.DT, "tile": info["tile"],
-                   "preroll": a.preroll, "timed_cycles": [start, start + a.steps],
+                   "preroll": a.preroll, "roofline_cycles": [a.preroll, a.preroll + R],
+                   "timed_cycles": [start, start + a.steps],
                    "timed_delivery_cycles": [start - 1, start + a.steps - 1],
                    "events_per_cycle": events / max(a.steps, 1),
                    "simulated_s": a.steps * netspec.DT, "parallelism": "single GPU"},
