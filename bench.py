@@ -231,17 +231,34 @@ def bench_sweep(a):
           flush=True)
     rng = np.random.default_rng(a.c3_seed + 1)
     sample = sorted(rng.choice(len(pts), size=min(a.c3_solo, len(pts)), replace=False).tolist())
-    # batched: every network in one simulation, built in parts of at most
-    # --c3-batch-syn synapses joined on the device (bsim_sim_create_chains),
-    # timed without recording; then the sampled networks again with the
-    # raster for the check
+    # batched: consecutive networks in simulations of at most --c3-sim-syn
+    # synapses (one simulation unless the sweep exceeds the device memory),
+    # each built in parts of at most --c3-batch-syn synapses joined on the
+    # device (bsim_sim_create_chains), timed without recording; then the
+    # sampled networks again with the raster for the check
     t0 = time.time()
     os.environ.setdefault("BSIM_PROGRESS", "1")     # one line per part of the build
-    _, secs_b, info = sweep.run_batch_parts(pts, steps, tile=tile, record=False,
+    groups, cur, cur_syn = [], [], 0
+    for p in pts:
+        ps = p.fan_in * sum(p.sizes[1:]) + p.drive_fan_in * p.sizes[0]
+        if cur and cur_syn + ps > a.c3_sim_syn:
+            groups.append(cur)
+            cur, cur_syn = [], 0
+        cur.append(p)
+        cur_syn += ps
+    groups.append(cur)
+    secs_b, events, ntiles, nsyn = 0.0, 0, 0, 0
+    for gi, grp in enumerate(groups):
+        _, sb, info = sweep.run_batch_parts(grp, steps, tile=tile, record=False,
                                             part_synapses=int(a.c3_batch_syn))
-    events, ntiles = info["events"], info["ntiles"]
-    print(f"[bench] batched: {secs_b:.3f}s for {len(pts)} networks in one simulation "
-          f"({ntiles} tiles, {info['s']} synapses, {events} events; built and run in "
+        secs_b += sb
+        events += info["events"]
+        ntiles += info["ntiles"]
+        nsyn += info["s"]
+        print(f"[bench] batched simulation {gi + 1}/{len(groups)}: {len(grp)} networks, "
+              f"{info['s']} synapses, {sb:.3f}s of stepping", file=sys.stderr, flush=True)
+    print(f"[bench] batched: {secs_b:.3f}s for {len(pts)} networks in {len(groups)} "
+          f"simulation(s) ({ntiles} tiles, {nsyn} synapses, {events} events; built and run in "
           f"{time.time() - t0:.0f}s)", file=sys.stderr, flush=True)
     rast, _, _ = sweep.run_batch([pts[i] for i in sample], steps, tile=tile, record=True)
     secs_s, exact = 0.0, 0
@@ -274,7 +291,7 @@ def bench_sweep(a):
                                f"w0 U[0.3, 1.5], inhibitory U[0.1, 0.4], drive U[5, 50] Hz, "
                                f"delays 1..U{{6..20}}, dt 0.1 ms, {steps} cycles",
                    "tile": tile, "autotune": {str(k): v for k, v in tt.items()},
-                   "simulations": 1, "synapses": info["s"],
+                   "simulations": len(groups), "synapses": nsyn,
                    "parallelism": "single GPU"},
         "one_at_a_time_networks_per_s": nps_s,
         "speedup": nps_b / nps_s,
@@ -450,6 +467,8 @@ def main():
     ap.add_argument("--c3-seed", type=int, default=7)
     ap.add_argument("--c3-batch-syn", type=float, default=1.5e9,
                     help="synapses per part of the batched simulation's host build")
+    ap.add_argument("--c3-sim-syn", type=float, default=2.5e9,
+                    help="synapses per batched simulation (device and host memory)")
     ap.add_argument("--c4-runs", type=int, default=4096)
     ap.add_argument("--c4-cands", type=int, default=1000000)
     ap.add_argument("--c4-resim", type=int, default=1024)

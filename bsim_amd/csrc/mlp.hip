@@ -12,18 +12,22 @@
  *   gather     minibatch rows of the resident data set -> X (bf16), Y (f32)
  *   forward    H_l = bf16(relu(H_{l-1} W_l^T + b_l)),  out = H_L W^T + b (f32)
  *   loss       dZ = bf16(2 (out - Y) / (B * out_dim)); loss and bias-gradient
- *              partials per 64 rows
- *   backward   dW_l = dZ_l^T H_{l-1} (split over the batch, fp32 partials);
- *              dZ_{l-1} = bf16(dZ_l W_l) where H_{l-1} > 0, with its
- *              bias-gradient partials per 64 rows
+ *              partials
+ *   backward   dZ_{l-1} = bf16(dZ_l W_l) where H_{l-1} > 0, with its
+ *              bias-gradient partials; dW_l = dZ_l^T H_{l-1} (split over the
+ *              batch, fp32 partials)
+ *   adam       sums every gradient's partials in a fixed order, then Adam
+ *              (Kingma & Ba, 2015) on the fp32 master, writes the bf16 copy
+ *              and its transpose
  * The input, hidden activations and activation gradients of a training step
  * are also written transposed ([unit][row]), so that the weight-gradient
  * GEMM reads both of its operands with the batch (its reduction index)
- * contiguous; likewise a transposed bf16 copy of every weight matrix, renewed
- * after each update, lets the activation-gradient GEMM read W^T with its
- * reduction index contiguous.
- *   adam       sums every gradient's partials in a fixed order, then Adam
- *              (Kingma & Ba, 2015) on the fp32 master, writes the bf16 copy
+ * contiguous; likewise the transposed bf16 copy of every weight matrix lets
+ * the activation-gradient products read W^T with their reduction index
+ * contiguous.
+ * Launches per step: the gather, one GEMM per layer forward, the loss, one
+ * GEMM per activation gradient, every weight gradient in one launch
+ * (mlp_gemm_partials) and Adam with the transposes in one (mlp_adam_tiles).
  * Every reduction has a fixed order, so results repeat bit for bit.
  *
  * There is no counterpart in the reference; the algorithm is the textbook
@@ -115,8 +119,9 @@ __device__ inline void store_tile(uint16_t *T, const Tile &t, uint32_t tid)
     *(uint4 *)(T + (r + 32) * LDS_STRIDE + kc) = t.v[1];
 }
 
+/* Output tile (bx, by) of split bz: C[by * 64 .. + 63][bx * 64 .. + 63]. */
 template <int EPI>
-__global__ __launch_bounds__(GEMM_THREADS) void mlp_gemm(GemmArgs g)
+__device__ inline void gemm_tile(const GemmArgs &g, int32_t bx, int32_t by, int32_t bz)
 {
     __shared__ __attribute__((aligned(16))) uint16_t As[2][BM * LDS_STRIDE];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[2][BN * LDS_STRIDE];
@@ -125,8 +130,8 @@ __global__ __launch_bounds__(GEMM_THREADS) void mlp_gemm(GemmArgs g)
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & (WAVE - 1), w = tid / WAVE;
     const uint32_t wm = w >> 1, wn = w & 1u;     /* wavefront's 32 x 32 quadrant */
-    const int32_t n0 = blockIdx.x * BN, m0 = blockIdx.y * BM;
-    const int32_t kbeg = blockIdx.z * g.K;
+    const int32_t n0 = bx * BN, m0 = by * BM;
+    const int32_t kbeg = bz * g.K;
     const int32_t nk = g.K / BK;
 
     f32x4 acc[2][2];
@@ -135,18 +140,33 @@ __global__ __launch_bounds__(GEMM_THREADS) void mlp_gemm(GemmArgs g)
 #pragma unroll
         for (int j = 0; j < 2; j++) acc[i][j] = f32x4{0.f, 0.f, 0.f, 0.f};
 
-    Tile ra = load_tile(g.A, g.lda, m0, kbeg, tid);
-    Tile rb = load_tile(g.B, g.ldb, n0, kbeg, tid);
-    store_tile(As[0], ra, tid);
-    store_tile(Bs[0], rb, tid);
+    /* k step s is multiplied from LDS buffer s & 1; at its start, step s+1
+     * (loaded during step s-1) moves from registers to the other buffer and
+     * the loads of step s+2 are issued, so that they are in flight during a
+     * whole step */
+    Tile ra, rb;
+    {
+        const Tile a0 = load_tile(g.A, g.lda, m0, kbeg, tid);
+        const Tile b0 = load_tile(g.B, g.ldb, n0, kbeg, tid);
+        if (nk > 1) {
+            ra = load_tile(g.A, g.lda, m0, kbeg + BK, tid);
+            rb = load_tile(g.B, g.ldb, n0, kbeg + BK, tid);
+        }
+        store_tile(As[0], a0, tid);
+        store_tile(Bs[0], b0, tid);
+    }
     __syncthreads();
 
     const uint32_t r16 = lane & 15u, kg = (lane >> 4) * 8u;
     for (int32_t s = 0; s < nk; s++) {
         const int cur = s & 1;
-        if (s + 1 < nk) {   /* next k step in flight while this one multiplies */
-            ra = load_tile(g.A, g.lda, m0, kbeg + (s + 1) * BK, tid);
-            rb = load_tile(g.B, g.ldb, n0, kbeg + (s + 1) * BK, tid);
+        if (s + 1 < nk) {
+            store_tile(As[cur ^ 1], ra, tid);
+            store_tile(Bs[cur ^ 1], rb, tid);
+            if (s + 2 < nk) {
+                ra = load_tile(g.A, g.lda, m0, kbeg + (s + 2) * BK, tid);
+                rb = load_tile(g.B, g.ldb, n0, kbeg + (s + 2) * BK, tid);
+            }
         }
 #pragma unroll
         for (int kk = 0; kk < BK; kk += 32) {
@@ -162,10 +182,6 @@ __global__ __launch_bounds__(GEMM_THREADS) void mlp_gemm(GemmArgs g)
 #pragma unroll
                 for (int j = 0; j < 2; j++)
                     acc[i][j] = __builtin_amdgcn_mfma_f32_16x16x32_bf16(a[i], b[j], acc[i][j], 0, 0, 0);
-        }
-        if (s + 1 < nk) {
-            store_tile(As[cur ^ 1], ra, tid);
-            store_tile(Bs[cur ^ 1], rb, tid);
         }
         __syncthreads();
     }
@@ -196,7 +212,7 @@ __global__ __launch_bounds__(GEMM_THREADS) void mlp_gemm(GemmArgs g)
                     g.out_bf[(int64_t)row * g.ldo + col] = h[r];
                     csum[j] += bf2f(h[r]);
                 } else {
-                    g.out_f[((int64_t)blockIdx.z * g.M + row) * g.ldo + col] = v;
+                    g.out_f[((int64_t)bz * g.M + row) * g.ldo + col] = v;
                 }
             }
             if ((EPI == EPI_RELU_BF16 || EPI == EPI_DRELU_BF16) && g.out_t) {
@@ -217,8 +233,33 @@ __global__ __launch_bounds__(GEMM_THREADS) void mlp_gemm(GemmArgs g)
             if (lane < 16) bsum[wm][wn * 32 + j * 16 + lane] = x;
         }
         __syncthreads();
-        if (tid < BN) g.bpart[(int64_t)blockIdx.y * g.N + n0 + tid] = bsum[0][tid] + bsum[1][tid];
+        if (tid < BN) g.bpart[(int64_t)by * g.N + n0 + tid] = bsum[0][tid] + bsum[1][tid];
     }
+}
+
+template <int EPI>
+__global__ __launch_bounds__(GEMM_THREADS) void mlp_gemm(GemmArgs g)
+{
+    gemm_tile<EPI>(g, blockIdx.x, blockIdx.y, blockIdx.z);
+}
+
+/* The weight-gradient GEMMs of every layer in one launch: workgroups
+ * first[l] .. first[l + 1] - 1 take layer l's tiles, split by split. */
+struct GemmGroup {
+    GemmArgs g[MAX_LAYERS];
+    int32_t first[MAX_LAYERS + 1];
+    int32_t n;
+};
+
+__global__ __launch_bounds__(GEMM_THREADS) void mlp_gemm_partials(GemmGroup a)
+{
+    int l = 0;
+    while (l + 1 < a.n && (int32_t)blockIdx.x >= a.first[l + 1]) l++;
+    const GemmArgs &g = a.g[l];
+    const int32_t t = (int32_t)blockIdx.x - a.first[l];
+    const int32_t tn = g.N / BN, tmn = tn * (g.M / BM);
+    const int32_t r = t % tmn;
+    gemm_tile<EPI_PARTIAL_F32>(g, r % tn, r / tn, t / tmn);
 }
 
 /* Loss of a minibatch: workgroup (x, y) takes rows [64 x, + 64) and output
@@ -308,26 +349,18 @@ struct AdamArgs {
     float *loss;
 };
 
-/* Adam on the fp32 master (m, v in fp32), the bf16 copy written alongside. */
-__global__ __launch_bounds__(256) void mlp_adam(AdamArgs a)
+/* Adam (Kingma & Ba, 2015) for the j-th parameter of segment sg: sums its
+ * gradient partials in a fixed order, updates the fp32 master (m, v in fp32)
+ * and writes the bf16 copy; returns the new bf16 value (0 when only the
+ * gradient is written). */
+__device__ inline uint16_t adam_elem(const AdamArgs &a, const AdamSeg &sg, int64_t j)
 {
-    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i == 0 && a.loss) {
-        float s = 0.f;
-        for (int32_t k = 0; k < a.nlpart; k++) s += a.lpart[k];
-        *a.loss = s * a.lscale;
-    }
-    int sidx = -1;
-    for (int s = 0; s < a.nseg; s++)
-        if (i >= a.seg[s].off && i < a.seg[s].off + a.seg[s].count) sidx = s;
-    if (sidx < 0) return;
-    const AdamSeg &sg = a.seg[sidx];
-    const int64_t j = i - sg.off;
+    const int64_t i = sg.off + j;
     float gsum = 0.f;
 #pragma unroll 8
     for (int32_t k = 0; k < sg.nparts; k++) gsum += sg.parts[(int64_t)k * sg.count + j];
     if (a.grad) a.grad[i] = gsum;
-    if (!a.update) return;
+    if (!a.update) return 0;
     const float m = a.b1 * a.m[i] + (1.f - a.b1) * gsum;
     const float v = a.b2 * a.v[i] + (1.f - a.b2) * (gsum * gsum);
     a.m[i] = m;
@@ -335,7 +368,59 @@ __global__ __launch_bounds__(256) void mlp_adam(AdamArgs a)
     const float mhat = m / a.c1, vhat = v / a.c2;
     const float p = a.p[i] - a.lr * (mhat / (sqrtf(vhat) + a.eps));
     a.p[i] = p;
-    a.pbf[i] = f2bf(p);
+    const uint16_t h = f2bf(p);
+    a.pbf[i] = h;
+    return h;
+}
+
+/* Adam over every parameter, with the transposed bf16 weights written
+ * alongside (the copy the activation-gradient products read): workgroups 0 .. wfirst[nlin] - 1 take
+ * one 64 x 64 tile of a weight matrix each and transpose its new bf16 values
+ * through LDS, the others 256 bias entries each. */
+struct AdamTileArgs {
+    AdamArgs a;
+    int32_t n[MAX_LAYERS], k[MAX_LAYERS];   /* W_l is [n][k] */
+    int32_t wfirst[MAX_LAYERS + 1];         /* first tile workgroup of W_l */
+    int32_t bfirst[MAX_LAYERS + 1];         /* first bias workgroup of layer l, after the tiles */
+    int32_t nlin;
+    uint16_t *pbf_t;                        /* W_l^T [k][n] at the offset of W_l */
+};
+
+constexpr int ADAM_THREADS = 1024;                   /* 4 parameters of a tile per thread */
+constexpr int ADAM_ROWS_STEP = ADAM_THREADS / 64;
+
+__global__ __launch_bounds__(ADAM_THREADS) void mlp_adam_tiles(AdamTileArgs t)
+{
+    __shared__ uint16_t tile[64][66];
+    const AdamArgs &a = t.a;
+    if (blockIdx.x == 0 && threadIdx.x == 0 && a.loss) {
+        float s = 0.f;
+        for (int32_t k = 0; k < a.nlpart; k++) s += a.lpart[k];
+        *a.loss = s * a.lscale;
+    }
+    const int32_t b = (int32_t)blockIdx.x;
+    if (b < t.wfirst[t.nlin]) {
+        int l = 0;
+        while (l + 1 < t.nlin && b >= t.wfirst[l + 1]) l++;
+        const int32_t ti = b - t.wfirst[l], tk = t.k[l] / 64;
+        const int32_t r0 = (ti / tk) * 64, c0 = (ti % tk) * 64;
+        const AdamSeg &sg = a.seg[2 * l];
+        const uint32_t x = threadIdx.x & 63u, y = threadIdx.x >> 6;
+#pragma unroll
+        for (uint32_t r = y; r < 64; r += ADAM_ROWS_STEP)
+            tile[r][x] = adam_elem(a, sg, (int64_t)(r0 + r) * t.k[l] + c0 + x);
+        if (!a.update) return;                    /* uniform */
+        __syncthreads();
+        uint16_t *wt = t.pbf_t + sg.off;
+        for (uint32_t c = y; c < 64; c += ADAM_ROWS_STEP) wt[(int64_t)(c0 + c) * t.n[l] + r0 + x] = tile[x][c];
+    } else {
+        const int32_t bb = b - t.wfirst[t.nlin];
+        int l = 0;
+        while (l + 1 < t.nlin && bb >= t.bfirst[l + 1]) l++;
+        const AdamSeg &sg = a.seg[2 * l + 1];
+        const int64_t j = (int64_t)(bb - t.bfirst[l]) * ADAM_THREADS + threadIdx.x;
+        if (j < sg.count) (void)adam_elem(a, sg, j);
+    }
 }
 
 /* Input conversion: f32 [n][in_dim] -> bf16 [n][kin] (zero padded). */
@@ -405,13 +490,13 @@ struct bsim_mlp {
     float *out = nullptr, *ybat = nullptr;
     uint16_t *dz[2]{};                /* activation gradients (ping-pong) */
     uint16_t *act_t[MAX_LAYERS]{};    /* act[l] transposed, [kdim[l]][batch] */
-    uint16_t *dz_t[2]{};              /* dz transposed, [width][batch] */
     float *wpart[MAX_LAYERS]{}, *bpart[MAX_LAYERS]{};
     float *lpart = nullptr, *loss = nullptr;
     uint16_t *xall = nullptr;
     float *yall = nullptr;
     int64_t ndata = 0;
     float *xstage = nullptr;          /* predict input staging (f32) */
+    uint16_t *dzt[MAX_LAYERS]{};      /* dZ of layer l's output, transposed [kdim[l+1]][batch] */
 };
 
 namespace {
@@ -480,34 +565,21 @@ int forward(bsim_mlp *m, int32_t batch, bool transposed)
 }
 
 /* loss + backward on a batch whose act[0] and ybat are set; gradient
- * partials left in wpart / bpart */
+ * partials left in wpart / bpart.  The activation gradients go down the
+ * layers one GEMM each, their transposed copies kept per layer, and then every
+ * weight gradient is formed in one launch. */
 int backward(bsim_mlp *m, int32_t batch)
 {
     const int L = m->nlin;
     const int32_t nout = m->kdim[L];
     const float scale = 2.0f / ((float)batch * (float)m->cfg.out_dim);
     hipLaunchKernelGGL(mlp_loss, dim3(batch / ROWS_PER_PART, nout / 64), dim3(256), 0, m->stream,
-                       m->out, m->ybat, m->dz[0], m->dz_t[0], (int64_t)batch, m->bpart[L - 1],
+                       m->out, m->ybat, m->dz[0], m->dzt[L - 1], (int64_t)batch, m->bpart[L - 1],
                        m->lpart, nout, m->cfg.out_dim, scale);
     hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_check(e, "loss launch");
     int cur = 0;
-    for (int l = L - 1; l >= 0; l--) {
-        /* dW_l [kdim[l+1]][kdim[l]] = dZ^T H_{l}: A(m = out unit, k = row) = dz_t[m][k],
-         * B(n = in unit, k = row) = act_t[l][n][k] */
-        GemmArgs gw{};
-        gw.A = m->dz_t[cur];
-        gw.lda = batch;
-        gw.B = m->act_t[l];
-        gw.ldb = batch;
-        gw.M = m->kdim[l + 1];
-        gw.N = m->kdim[l];
-        gw.K = batch / KSPLIT;
-        gw.out_f = m->wpart[l];
-        gw.ldo = m->kdim[l];
-        e = gemm<EPI_PARTIAL_F32>(gw, KSPLIT, m->stream);
-        if (e != hipSuccess) return hip_check(e, "weight-gradient launch");
-        if (l == 0) break;
+    for (int l = L - 1; l >= 1; l--) {
         /* dZ_{l-1} [batch][kdim[l]] = (dZ W_l) where H > 0:
          * B(n = in unit, k = out unit) = W[k][n] = W^T[n][k] */
         GemmArgs gx{};
@@ -522,13 +594,31 @@ int backward(bsim_mlp *m, int32_t batch)
         gx.ldo = m->kdim[l];
         gx.mask = m->act[l];
         gx.bpart = m->bpart[l - 1];
-        gx.out_t = m->dz_t[cur ^ 1];
+        gx.out_t = m->dzt[l - 1];
         gx.ldt = batch;
         e = gemm<EPI_DRELU_BF16>(gx, 1, m->stream);
         if (e != hipSuccess) return hip_check(e, "activation-gradient launch");
         cur ^= 1;
     }
-    return BSIM_OK;
+    /* dW_l [kdim[l+1]][kdim[l]] = dZ_l^T H_l: A(m = out unit, k = row) = dzt[l][m][k],
+     * B(n = in unit, k = row) = act_t[l][n][k]; every layer in one launch */
+    GemmGroup g{};
+    g.n = L;
+    for (int l = 0; l < L; l++) {
+        GemmArgs &x = g.g[l];
+        x.A = m->dzt[l];
+        x.lda = batch;
+        x.B = m->act_t[l];
+        x.ldb = batch;
+        x.M = m->kdim[l + 1];
+        x.N = m->kdim[l];
+        x.K = batch / KSPLIT;
+        x.out_f = m->wpart[l];
+        x.ldo = m->kdim[l];
+        g.first[l + 1] = g.first[l] + (x.M / BM) * (x.N / BN) * KSPLIT;
+    }
+    hipLaunchKernelGGL(mlp_gemm_partials, dim3(g.first[L]), dim3(GEMM_THREADS), 0, m->stream, g);
+    return hip_check(hipGetLastError(), "weight-gradient launch");
 }
 
 int adam(bsim_mlp *m, int32_t batch, bool update, float lr, float *grad_out, float *loss_dev)
@@ -556,8 +646,18 @@ int adam(bsim_mlp *m, int32_t batch, bool update, float lr, float *grad_out, flo
     a.nlpart = batch / ROWS_PER_PART * (m->kdim[m->nlin] / 64);
     a.lscale = 1.0f / ((float)batch * (float)m->cfg.out_dim);
     a.loss = loss_dev;
-    const int64_t blocks = (m->nparam + 255) / 256;
-    hipLaunchKernelGGL(mlp_adam, dim3((uint32_t)blocks), dim3(256), 0, m->stream, a);
+    AdamTileArgs ta{};
+    ta.a = a;
+    ta.nlin = m->nlin;
+    ta.pbf_t = m->pbf_t;
+    for (int l = 0; l < m->nlin; l++) {
+        ta.n[l] = m->kdim[l + 1];
+        ta.k[l] = m->kdim[l];
+        ta.wfirst[l + 1] = ta.wfirst[l] + (ta.n[l] / 64) * (ta.k[l] / 64);
+        ta.bfirst[l + 1] = ta.bfirst[l] + (ta.n[l] + ADAM_THREADS - 1) / ADAM_THREADS;
+    }
+    hipLaunchKernelGGL(mlp_adam_tiles, dim3((uint32_t)(ta.wfirst[m->nlin] + ta.bfirst[m->nlin])),
+                       dim3(ADAM_THREADS), 0, m->stream, ta);
     const hipError_t e = hipGetLastError();
     if (e != hipSuccess) return hip_check(e, "adam launch");
     if (update) m->step++;
@@ -650,12 +750,11 @@ bsim_mlp *bsim_mlp_create(const bsim_mlp_config *cfg)
     for (int l = 0; l <= m->nlin; l++) wmax = std::max(wmax, m->kdim[l]);
     if (!rc) rc = dalloc(m, &m->dz[0], B * wmax);
     if (!rc) rc = dalloc(m, &m->dz[1], B * wmax);
-    if (!rc) rc = dalloc(m, &m->dz_t[0], B * wmax);
-    if (!rc) rc = dalloc(m, &m->dz_t[1], B * wmax);
     for (int l = 0; l < m->nlin && !rc; l++) rc = dalloc(m, &m->act_t[l], B * m->kdim[l]);
     for (int l = 0; l < m->nlin && !rc; l++) {
         rc = dalloc(m, &m->wpart[l], (int64_t)KSPLIT * m->kdim[l + 1] * m->kdim[l]);
         if (!rc) rc = dalloc(m, &m->bpart[l], B / ROWS_PER_PART * m->kdim[l + 1]);
+        if (!rc) rc = dalloc(m, &m->dzt[l], B * m->kdim[l + 1]);
     }
     if (!rc) rc = dalloc(m, &m->lpart, B / ROWS_PER_PART * (m->kdim[m->nlin] / 64));
     if (!rc) rc = dalloc(m, &m->loss, 1);
@@ -786,8 +885,7 @@ int train_steps(bsim_mlp *m, const int32_t *rdev, float *ldev, int32_t steps, in
         rc = hip_check(hipGetLastError(), "gather launch");
         if (!rc) rc = forward(m, batch, true);
         if (!rc) rc = backward(m, batch);
-        if (!rc) rc = adam(m, batch, true, lr, nullptr, ldev + s);
-        if (!rc) rc = transpose_weights(m);
+        if (!rc) rc = adam(m, batch, true, lr, nullptr, ldev + s);   /* writes W^T too */
     }
     const int src = hip_check(hipStreamSynchronize(m->stream), "sync");
     return rc ? rc : src;
