@@ -25,9 +25,11 @@
  * contiguous; likewise the transposed bf16 copy of every weight matrix lets
  * the activation-gradient products read W^T with their reduction index
  * contiguous.
- * Launches per step: the gather, one GEMM per layer forward, the loss, one
- * GEMM per activation gradient, every weight gradient in one launch
- * (mlp_gemm_partials) and Adam with the transposes in one (mlp_adam_tiles).
+ * Launches per step: one GEMM per layer forward (the first gathers its
+ * minibatch rows from the resident set, the last forms the loss and dZ of the
+ * output in its epilogue), one GEMM per activation gradient, every weight
+ * gradient in one launch (mlp_gemm_partials) and Adam with the transposes in
+ * one (mlp_adam_tiles).
  * Every reduction has a fixed order, so results repeat bit for bit.
  *
  * There is no counterpart in the reference; the algorithm is the textbook
@@ -77,7 +79,7 @@ __host__ __device__ inline float bf2f(uint16_t h)
     return x;
 }
 
-enum Epi { EPI_RELU_BF16 = 0, EPI_LINEAR_F32 = 1, EPI_DRELU_BF16 = 2, EPI_PARTIAL_F32 = 3 };
+enum Epi { EPI_RELU_BF16 = 0, EPI_LINEAR_F32 = 1, EPI_DRELU_BF16 = 2, EPI_PARTIAL_F32 = 3, EPI_LOSS = 4 };
 
 /* C[m][n] = sum_k A[m * lda + k] B[n * ldb + k] over this workgroup's k
  * range (both operands k-contiguous). */
@@ -93,8 +95,15 @@ struct GemmArgs {
     int64_t ldo;
     const uint16_t *mask;   /* EPI_DRELU_BF16: keep where mask[m * ldo + n] > 0 */
     float *bpart;           /* EPI_DRELU_BF16: bias-gradient partials [M / 64][N] */
-    uint16_t *out_t;        /* EPI_RELU_BF16, EPI_DRELU_BF16: transposed copy [N][ldt] or null */
+    uint16_t *out_t;        /* EPI_RELU_BF16, EPI_DRELU_BF16, EPI_LOSS: transposed copy [N][ldt] or null */
     int64_t ldt;
+    const int32_t *arow;    /* row m of A is row arow[m] of the array A (null: row m) */
+    uint16_t *a_t;          /* with K == BK: workgroups of column tile 0 write A^T [K][ldt] */
+    const float *y;         /* EPI_LOSS: targets, row m at y[(yrow ? yrow[m] : m) * ny] */
+    const int32_t *yrow;
+    int32_t ny;             /* EPI_LOSS: valid output columns */
+    float scale;            /* EPI_LOSS: dZ = bf16(scale * (C + bias - y)) */
+    float *lpart;           /* EPI_LOSS: squared-error sum of each 64 x 64 tile [M / 64][N / 64] */
 };
 
 /* One k step of a 64-row operand: thread tid holds rows tid / 8 and
@@ -103,12 +112,15 @@ struct Tile {
     uint4 v[2];
 };
 
-__device__ inline Tile load_tile(const uint16_t *P, int64_t ld, int32_t r0, int32_t k0, uint32_t tid)
+__device__ inline Tile load_tile(const uint16_t *P, int64_t ld, int32_t r0, int32_t k0, uint32_t tid,
+                                 const int32_t *rowmap = nullptr)
 {
     const uint32_t r = tid >> 3, kc = (tid & 7u) * 8u;
+    const int64_t ra = rowmap ? (int64_t)rowmap[r0 + r] : (int64_t)(r0 + r);
+    const int64_t rb = rowmap ? (int64_t)rowmap[r0 + r + 32] : (int64_t)(r0 + r + 32);
     Tile t;
-    t.v[0] = *(const uint4 *)(P + (int64_t)(r0 + r) * ld + k0 + kc);
-    t.v[1] = *(const uint4 *)(P + (int64_t)(r0 + r + 32) * ld + k0 + kc);
+    t.v[0] = *(const uint4 *)(P + ra * ld + k0 + kc);
+    t.v[1] = *(const uint4 *)(P + rb * ld + k0 + kc);
     return t;
 }
 
@@ -126,6 +138,7 @@ __device__ inline void gemm_tile(const GemmArgs &g, int32_t bx, int32_t by, int3
     __shared__ __attribute__((aligned(16))) uint16_t As[2][BM * LDS_STRIDE];
     __shared__ __attribute__((aligned(16))) uint16_t Bs[2][BN * LDS_STRIDE];
     __shared__ float bsum[2][BN];
+    __shared__ float lwave[GEMM_THREADS / WAVE];
 
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & (WAVE - 1), w = tid / WAVE;
@@ -146,16 +159,48 @@ __device__ inline void gemm_tile(const GemmArgs &g, int32_t bx, int32_t by, int3
      * whole step */
     Tile ra, rb;
     {
-        const Tile a0 = load_tile(g.A, g.lda, m0, kbeg, tid);
+        const Tile a0 = load_tile(g.A, g.lda, m0, kbeg, tid, g.arow);
         const Tile b0 = load_tile(g.B, g.ldb, n0, kbeg, tid);
         if (nk > 1) {
-            ra = load_tile(g.A, g.lda, m0, kbeg + BK, tid);
+            ra = load_tile(g.A, g.lda, m0, kbeg + BK, tid, g.arow);
             rb = load_tile(g.B, g.ldb, n0, kbeg + BK, tid);
         }
         store_tile(As[0], a0, tid);
         store_tile(Bs[0], b0, tid);
     }
+    /* the epilogue's other operands, loaded while the products run: the
+     * activation mask (EPI_DRELU_BF16) or the targets (EPI_LOSS) of this
+     * lane's 16 outputs (rows m0 + 32 wm + 16 i + 4 (lane / 16) + r, column
+     * n0 + 32 wn + 16 j + lane % 16) */
+    uint16_t mk[2][2][4];
+    float yv[2][2][4];
+    if (EPI == EPI_DRELU_BF16 || EPI == EPI_LOSS) {
+        const uint32_t rq0 = (lane >> 4) * 4u, c16 = lane & 15u;
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int32_t row = m0 + (int32_t)(wm * 32 + i * 16 + rq0) + r;
+                    const int32_t col = n0 + (int32_t)(wn * 32 + j * 16 + c16);
+                    if (EPI == EPI_DRELU_BF16) {
+                        mk[i][j][r] = g.mask[(int64_t)row * g.ldo + col];
+                    } else {
+                        const int64_t yr = g.yrow ? (int64_t)g.yrow[row] : (int64_t)row;
+                        yv[i][j][r] = col < g.ny ? g.y[yr * g.ny + col] : 0.f;
+                    }
+                }
+    }
     __syncthreads();
+    /* the first layer of a training step: A's rows are gathered, and the
+     * workgroups of column tile 0 also write them transposed (one k step) */
+    if (g.a_t && bx == 0 && nk == 1) {
+        for (uint32_t x = tid; x < (uint32_t)(BM * BK); x += GEMM_THREADS) {
+            const uint32_t k = x / BM, r = x % BM;
+            g.a_t[(int64_t)(kbeg + (int32_t)k) * g.ldt + m0 + (int32_t)r] = As[0][r * LDS_STRIDE + k];
+        }
+    }
 
     const uint32_t r16 = lane & 15u, kg = (lane >> 4) * 8u;
     for (int32_t s = 0; s < nk; s++) {
@@ -164,7 +209,7 @@ __device__ inline void gemm_tile(const GemmArgs &g, int32_t bx, int32_t by, int3
             store_tile(As[cur ^ 1], ra, tid);
             store_tile(Bs[cur ^ 1], rb, tid);
             if (s + 2 < nk) {
-                ra = load_tile(g.A, g.lda, m0, kbeg + (s + 2) * BK, tid);
+                ra = load_tile(g.A, g.lda, m0, kbeg + (s + 2) * BK, tid, g.arow);
                 rb = load_tile(g.B, g.ldb, n0, kbeg + (s + 2) * BK, tid);
             }
         }
@@ -189,6 +234,7 @@ __device__ inline void gemm_tile(const GemmArgs &g, int32_t bx, int32_t by, int3
     /* lane holds C[row0 + 4 * (lane / 16) + r][col0 + lane % 16], r = 0..3 */
     const uint32_t rq = (lane >> 4) * 4u;
     float csum[2] = {0.f, 0.f};
+    float lsum = 0.f;
 #pragma unroll
     for (int i = 0; i < 2; i++) {
 #pragma unroll
@@ -207,15 +253,26 @@ __device__ inline void gemm_tile(const GemmArgs &g, int32_t bx, int32_t by, int3
                 } else if (EPI == EPI_LINEAR_F32) {
                     g.out_f[(int64_t)row * g.ldo + col] = v + g.bias[col];
                 } else if (EPI == EPI_DRELU_BF16) {
-                    const uint16_t m = g.mask[(int64_t)row * g.ldo + col];
+                    const uint16_t m = mk[i][j][r];
                     h[r] = (m != 0 && !(m & 0x8000u)) ? f2bf(v) : (uint16_t)0;
+                    g.out_bf[(int64_t)row * g.ldo + col] = h[r];
+                    csum[j] += bf2f(h[r]);
+                } else if (EPI == EPI_LOSS) {
+                    /* the output (as EPI_LINEAR_F32 rounds it), its error and dZ */
+                    const float out = v + g.bias[col];
+                    h[r] = 0;
+                    if (col < g.ny) {
+                        const float diff = out - yv[i][j][r];
+                        lsum += diff * diff;
+                        h[r] = f2bf(g.scale * diff);
+                    }
                     g.out_bf[(int64_t)row * g.ldo + col] = h[r];
                     csum[j] += bf2f(h[r]);
                 } else {
                     g.out_f[((int64_t)bz * g.M + row) * g.ldo + col] = v;
                 }
             }
-            if ((EPI == EPI_RELU_BF16 || EPI == EPI_DRELU_BF16) && g.out_t) {
+            if ((EPI == EPI_RELU_BF16 || EPI == EPI_DRELU_BF16 || EPI == EPI_LOSS) && g.out_t) {
                 /* 4 consecutive rows of one column: one 8-byte store */
                 const uint2 q = {(uint32_t)h[0] | ((uint32_t)h[1] << 16),
                                  (uint32_t)h[2] | ((uint32_t)h[3] << 16)};
@@ -223,7 +280,7 @@ __device__ inline void gemm_tile(const GemmArgs &g, int32_t bx, int32_t by, int3
             }
         }
     }
-    if (EPI == EPI_DRELU_BF16) {
+    if (EPI == EPI_DRELU_BF16 || EPI == EPI_LOSS) {
         /* column sums over the workgroup's 64 rows, in a fixed order */
 #pragma unroll
         for (int j = 0; j < 2; j++) {
@@ -232,8 +289,17 @@ __device__ inline void gemm_tile(const GemmArgs &g, int32_t bx, int32_t by, int3
             x += __shfl_xor(x, 32);
             if (lane < 16) bsum[wm][wn * 32 + j * 16 + lane] = x;
         }
+        if (EPI == EPI_LOSS) {
+            /* the tile's squared error: lane, wavefront, then the four
+             * wavefronts, in a fixed order */
+#pragma unroll
+            for (int o = 1; o < WAVE; o <<= 1) lsum += __shfl_xor(lsum, o);
+            if (lane == 0) lwave[w] = lsum;
+        }
         __syncthreads();
         if (tid < BN) g.bpart[(int64_t)by * g.N + n0 + tid] = bsum[0][tid] + bsum[1][tid];
+        if (EPI == EPI_LOSS && tid == 0)
+            g.lpart[(int64_t)by * (g.N / BN) + bx] = ((lwave[0] + lwave[1]) + lwave[2]) + lwave[3];
     }
 }
 
@@ -260,73 +326,6 @@ __global__ __launch_bounds__(GEMM_THREADS) void mlp_gemm_partials(GemmGroup a)
     const int32_t tn = g.N / BN, tmn = tn * (g.M / BM);
     const int32_t r = t % tmn;
     gemm_tile<EPI_PARTIAL_F32>(g, r % tn, r / tn, t / tmn);
-}
-
-/* Loss of a minibatch: workgroup (x, y) takes rows [64 x, + 64) and output
- * columns [64 y, + 64); thread (c, q) = (tid % 64, tid / 64) takes column
- * 64 y + c over rows 64 x + 16 q .. + 15.  dZ = bf16(scale * (out - y)) (0 in
- * padded columns), written row-major and transposed; partial sums of
- * (out - y)^2 per workgroup and of the bf16 dZ per column and 64 rows, each in
- * a fixed order. */
-__global__ __launch_bounds__(256) void mlp_loss(const float *out, const float *y, uint16_t *dz,
-                                                uint16_t *dz_t, int64_t ldt, float *bpart,
-                                                float *lpart, int32_t npad, int32_t nvalid,
-                                                float scale)
-{
-    __shared__ float cs[4][64];
-    __shared__ float red[256];
-    const uint32_t tid = threadIdx.x, c = tid & 63u, q = tid >> 6;
-    const int32_t col = blockIdx.y * 64 + c;
-    const int32_t r0 = blockIdx.x * ROWS_PER_PART + q * 16;
-    float lsum = 0.f, csum = 0.f;
-    uint16_t h[16];
-#pragma unroll
-    for (int r = 0; r < 16; r++) {
-        const int32_t row = r0 + r;
-        h[r] = 0;
-        if (col < nvalid) {
-            const float diff = out[(int64_t)row * npad + col] - y[(int64_t)row * nvalid + col];
-            lsum += diff * diff;
-            h[r] = f2bf(scale * diff);
-        }
-        dz[(int64_t)row * npad + col] = h[r];
-        csum += bf2f(h[r]);
-    }
-    uint4 *t = (uint4 *)(dz_t + (int64_t)col * ldt + r0);
-#pragma unroll
-    for (int k = 0; k < 2; k++) {
-        uint32_t w[4];
-#pragma unroll
-        for (int j = 0; j < 4; j++)
-            w[j] = (uint32_t)h[8 * k + 2 * j] | ((uint32_t)h[8 * k + 2 * j + 1] << 16);
-        t[k] = uint4{w[0], w[1], w[2], w[3]};
-    }
-    cs[q][c] = csum;
-    red[tid] = lsum;
-    __syncthreads();
-    if (q == 0) bpart[(int64_t)blockIdx.x * npad + col] = ((cs[0][c] + cs[1][c]) + cs[2][c]) + cs[3][c];
-    for (int s = 128; s > 0; s >>= 1) {
-        if ((int)tid < s) red[tid] += red[tid + s];
-        __syncthreads();
-    }
-    if (tid == 0) lpart[(int64_t)blockIdx.y * gridDim.x + blockIdx.x] = red[0];
-}
-
-/* Minibatch gather: X rows (bf16, 64 wide) and Y rows of the resident set. */
-__global__ __launch_bounds__(256) void mlp_gather(const uint16_t *xall, const float *yall,
-                                                  const int32_t *rows, uint16_t *x, uint16_t *x_t,
-                                                  float *y, int32_t batch, int32_t kin,
-                                                  int32_t nvalid)
-{
-    const int32_t b = blockIdx.x;
-    const int64_t src = rows[b];
-    for (int32_t c = threadIdx.x; c < kin; c += blockDim.x) {
-        const uint16_t v = xall[src * kin + c];
-        x[(int64_t)b * kin + c] = v;
-        x_t[(int64_t)c * batch + b] = v;
-    }
-    for (int32_t c = threadIdx.x; c < nvalid; c += blockDim.x)
-        y[(int64_t)b * nvalid + c] = yall[src * nvalid + c];
 }
 
 struct AdamSeg {
@@ -534,13 +533,21 @@ double hash_uniform(uint64_t seed, uint64_t i)
     return (double)(z >> 11) * (2.0 / 9007199254740992.0) - 1.0;
 }
 
-/* forward pass on act[0] (batch rows); fp32 output in m->out; with
- * transposed, the hidden activations are also written to act_t */
-int forward(bsim_mlp *m, int32_t batch, bool transposed)
+/* Forward pass of a batch.  predict (train false): from act[0], fp32 output
+ * in m->out.  train: the hidden activations also written transposed, and the
+ * output layer forms the loss partials, dZ of the output (row-major and
+ * transposed) and its bias partials against the targets (EPI_LOSS); with
+ * rows, layer 0 gathers its inputs from the resident set (writing them
+ * transposed) and the targets are the resident ones, else act[0] / act_t[0]
+ * and ybat hold them. */
+int forward(bsim_mlp *m, int32_t batch, bool train, const int32_t *rows = nullptr)
 {
-    for (int l = 0; l < m->nlin; l++) {
+    const int L = m->nlin;
+    for (int l = 0; l < L; l++) {
         GemmArgs g{};
-        g.A = m->act[l];
+        g.A = (l == 0 && rows) ? m->xall : m->act[l];
+        g.arow = l == 0 ? rows : nullptr;
+        g.a_t = (l == 0 && rows) ? m->act_t[0] : nullptr;
         g.lda = m->kdim[l];
         g.B = m->pbf + m->woff[l];
         g.ldb = m->kdim[l];
@@ -549,35 +556,39 @@ int forward(bsim_mlp *m, int32_t batch, bool transposed)
         g.K = m->kdim[l];
         g.bias = m->p + m->boff[l];
         g.ldo = m->kdim[l + 1];
+        g.ldt = batch;
         hipError_t e;
-        if (l + 1 < m->nlin) {
+        if (l + 1 < L) {
             g.out_bf = m->act[l + 1];
-            g.out_t = transposed ? m->act_t[l + 1] : nullptr;
-            g.ldt = batch;
+            g.out_t = train ? m->act_t[l + 1] : nullptr;
             e = gemm<EPI_RELU_BF16>(g, 1, m->stream);
-        } else {
+        } else if (!train) {
             g.out_f = m->out;
             e = gemm<EPI_LINEAR_F32>(g, 1, m->stream);
+        } else {
+            g.out_bf = m->dz[0];
+            g.out_t = m->dzt[L - 1];
+            g.bpart = m->bpart[L - 1];
+            g.lpart = m->lpart;
+            g.y = rows ? m->yall : m->ybat;
+            g.yrow = rows;
+            g.ny = m->cfg.out_dim;
+            g.scale = 2.0f / ((float)batch * (float)m->cfg.out_dim);
+            e = gemm<EPI_LOSS>(g, 1, m->stream);
         }
         if (e != hipSuccess) return hip_check(e, "forward launch");
     }
     return BSIM_OK;
 }
 
-/* loss + backward on a batch whose act[0] and ybat are set; gradient
- * partials left in wpart / bpart.  The activation gradients go down the
- * layers one GEMM each, their transposed copies kept per layer, and then every
- * weight gradient is formed in one launch. */
+/* Backward pass of a batch after a training forward pass (which formed dZ
+ * of the output): gradient partials left in wpart / bpart.  The activation
+ * gradients go down the layers one GEMM each, their transposed copies kept
+ * per layer, and then every weight gradient is formed in one launch. */
 int backward(bsim_mlp *m, int32_t batch)
 {
     const int L = m->nlin;
-    const int32_t nout = m->kdim[L];
-    const float scale = 2.0f / ((float)batch * (float)m->cfg.out_dim);
-    hipLaunchKernelGGL(mlp_loss, dim3(batch / ROWS_PER_PART, nout / 64), dim3(256), 0, m->stream,
-                       m->out, m->ybat, m->dz[0], m->dzt[L - 1], (int64_t)batch, m->bpart[L - 1],
-                       m->lpart, nout, m->cfg.out_dim, scale);
-    hipError_t e = hipGetLastError();
-    if (e != hipSuccess) return hip_check(e, "loss launch");
+    hipError_t e;
     int cur = 0;
     for (int l = L - 1; l >= 1; l--) {
         /* dZ_{l-1} [batch][kdim[l]] = (dZ W_l) where H > 0:
@@ -879,11 +890,8 @@ int train_steps(bsim_mlp *m, const int32_t *rdev, float *ldev, int32_t steps, in
 {
     int rc = BSIM_OK;
     for (int32_t s = 0; s < steps && !rc; s++) {
-        hipLaunchKernelGGL(mlp_gather, dim3(batch), dim3(256), 0, m->stream, m->xall, m->yall,
-                           rdev + (int64_t)s * batch, m->act[0], m->act_t[0], m->ybat, batch,
-                           m->kdim[0], m->cfg.out_dim);
-        rc = hip_check(hipGetLastError(), "gather launch");
-        if (!rc) rc = forward(m, batch, true);
+        /* the minibatch is gathered by the first layer's GEMM and the loss */
+        rc = forward(m, batch, true, rdev + (int64_t)s * batch);
         if (!rc) rc = backward(m, batch);
         if (!rc) rc = adam(m, batch, true, lr, nullptr, ldev + s);   /* writes W^T too */
     }
