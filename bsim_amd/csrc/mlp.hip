@@ -372,6 +372,47 @@ __device__ inline uint16_t adam_elem(const AdamArgs &a, const AdamSeg &sg, int64
     return h;
 }
 
+/* adam_elem for E parameters j[0..E-1] of one segment at once: every load
+ * of the E is issued before the first store (the same arithmetic, element by
+ * element). */
+template <int E>
+__device__ inline void adam_elems(const AdamArgs &a, const AdamSeg &sg, const int64_t (&j)[E], uint16_t (&h)[E])
+{
+    float g[E], mo[E], vo[E], po[E];
+#pragma unroll
+    for (int e = 0; e < E; e++) g[e] = 0.f;
+    for (int32_t k = 0; k < sg.nparts; k++)
+#pragma unroll
+        for (int e = 0; e < E; e++) g[e] += sg.parts[(int64_t)k * sg.count + j[e]];
+    if (a.grad)
+#pragma unroll
+        for (int e = 0; e < E; e++) a.grad[sg.off + j[e]] = g[e];
+    if (!a.update) {
+#pragma unroll
+        for (int e = 0; e < E; e++) h[e] = 0;
+        return;
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        mo[e] = a.m[sg.off + j[e]];
+        vo[e] = a.v[sg.off + j[e]];
+        po[e] = a.p[sg.off + j[e]];
+    }
+#pragma unroll
+    for (int e = 0; e < E; e++) {
+        const int64_t i = sg.off + j[e];
+        const float m = a.b1 * mo[e] + (1.f - a.b1) * g[e];
+        const float v = a.b2 * vo[e] + (1.f - a.b2) * (g[e] * g[e]);
+        a.m[i] = m;
+        a.v[i] = v;
+        const float mhat = m / a.c1, vhat = v / a.c2;
+        const float p = po[e] - a.lr * (mhat / (sqrtf(vhat) + a.eps));
+        a.p[i] = p;
+        h[e] = f2bf(p);
+        a.pbf[i] = h[e];
+    }
+}
+
 /* Adam over every parameter, with the transposed bf16 weights written
  * alongside (the copy the activation-gradient products read): workgroups 0 .. wfirst[nlin] - 1 take
  * one 64 x 64 tile of a weight matrix each and transpose its new bf16 values
@@ -405,9 +446,14 @@ __global__ __launch_bounds__(ADAM_THREADS) void mlp_adam_tiles(AdamTileArgs t)
         const int32_t r0 = (ti / tk) * 64, c0 = (ti % tk) * 64;
         const AdamSeg &sg = a.seg[2 * l];
         const uint32_t x = threadIdx.x & 63u, y = threadIdx.x >> 6;
+        constexpr int E = 64 / ADAM_ROWS_STEP;
+        int64_t j[E];
+        uint16_t h[E];
 #pragma unroll
-        for (uint32_t r = y; r < 64; r += ADAM_ROWS_STEP)
-            tile[r][x] = adam_elem(a, sg, (int64_t)(r0 + r) * t.k[l] + c0 + x);
+        for (int e = 0; e < E; e++) j[e] = (int64_t)(r0 + (int32_t)y + e * ADAM_ROWS_STEP) * t.k[l] + c0 + x;
+        adam_elems<E>(a, sg, j, h);
+#pragma unroll
+        for (int e = 0; e < E; e++) tile[y + e * ADAM_ROWS_STEP][x] = h[e];
         if (!a.update) return;                    /* uniform */
         __syncthreads();
         uint16_t *wt = t.pbf_t + sg.off;
