@@ -354,7 +354,7 @@ int build_layout(const HostNet &net, float dt, int32_t tile, Layout &L, std::str
      * costs its own source ranges and row-table vectors, so a tile whose
      * targets have more than kMaxSeg segments is rejected (the first synapse
      * that opens segment kMaxSeg is reported). */
-#pragma omp parallel for schedule(dynamic, 1) reduction(min : first_unordered)
+#pragma omp parallel for schedule(dynamic, 1) reduction(min : first_unordered) reduction(max : fail)
     for (int32_t t = 0; t < NT; t++) {
         const int64_t lo = L.tile_syn[t], hi = L.tile_syn[t + 1], m = hi - lo;
         /* row key: segment (5 bits) | pre (31 bits) | delay (6 bits) |
