@@ -335,6 +335,23 @@ def test_edge_networks_vs_oracle(kind, tile):
 
 
 @pytest.mark.gpu
+def test_device_allocation_failure_is_reported():
+    """A recording window larger than the device (2^31 - 1 cycles of a
+    10,000-neuron raster, 2.7 TB): bsim_sim_create reports BSIM_E_NOMEM's
+    failure instead of crashing and frees what it allocated; the device then
+    runs the next simulation as usual (bit-exact against the oracle)."""
+    b = netspec.Builder()
+    rows = np.tile(netspec.lif_row(0.0, 0.0, 0.0, 0.02, 0.02, 2e-3, 5e-3, 8e-3, 1.0, 0.0), (10000, 1))
+    b.population(rows)
+    big = api.Network.from_spec(b.build())
+    with pytest.raises(api.BsimError, match="out of memory"):
+        api.Simulation(big, DT, rec_steps=2 ** 31 - 1)
+    spec = _edge_net("all_inhibitory")
+    r = run_gpu(spec, 200, [0, spec.n - 1])
+    assert_same(r, oracle.run(spec, DT, 200, rec_ids=[0, spec.n - 1]), "after a failed create")
+
+
+@pytest.mark.gpu
 def test_zero_steps_and_finish():
     spec = _edge_net("one_neuron")
     sim = api.Simulation(api.Network.from_spec(spec), DT, rec_steps=10)
