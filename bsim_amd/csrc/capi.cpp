@@ -299,7 +299,10 @@ static T *dev_alloc(bsim_sim *sim, size_t count, hipError_t &e)
     void *p = nullptr;
     const size_t bytes = (count ? count : 1) * sizeof(T);
     e = hipMalloc(&p, bytes);
-    if (e != hipSuccess) return nullptr;
+    if (e != hipSuccess) {
+        (void)hipGetLastError();   /* the failure is reported here, not by a later launch check */
+        return nullptr;
+    }
     sim->allocs.push_back(p);
     sim->device_bytes += (int64_t)bytes;
     return (T *)p;
@@ -390,6 +393,7 @@ bsim_sim *bsim_sim_create_chains(const bsim_chain_params *p, int32_t count, floa
     const int64_t sbytes = std::max<int64_t>(ns, 1) * (int64_t)sizeof(bsim_syn);
     hipError_t e = hipMalloc((void **)&dsyn, (size_t)sbytes);
     if (e != hipSuccess) {
+        (void)hipGetLastError();
         fail(e == hipErrorOutOfMemory ? BSIM_E_NOMEM : BSIM_E_HIP,
              std::string("bsim_sim_create_chains: ") + hipGetErrorString(e));
         return nullptr;

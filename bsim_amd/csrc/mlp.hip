@@ -511,6 +511,7 @@ template <int EPI>
 hipError_t gemm(const GemmArgs &g, int32_t splits, hipStream_t st)
 {
     const dim3 grid(g.N / BN, g.M / BM, splits);
+    (void)hipGetLastError();   /* the check after the launch reports it alone */
     hipLaunchKernelGGL((mlp_gemm<EPI>), grid, dim3(GEMM_THREADS), 0, st, g);
     return hipGetLastError();
 }
@@ -563,7 +564,10 @@ int dalloc(bsim_mlp *m, T **p, int64_t count)
 {
     void *q = nullptr;
     const hipError_t e = hipMalloc(&q, (size_t)std::max<int64_t>(count, 1) * sizeof(T));
-    if (e != hipSuccess) return hip_check(e, "hipMalloc");
+    if (e != hipSuccess) {
+        (void)hipGetLastError();   /* the failure is reported here, not by a later launch check */
+        return hip_check(e, "hipMalloc");
+    }
     m->allocs.push_back(q);
     *p = (T *)q;
     return BSIM_OK;
@@ -674,6 +678,7 @@ int backward(bsim_mlp *m, int32_t batch)
         x.ldo = m->kdim[l];
         g.first[l + 1] = g.first[l] + (x.M / BM) * (x.N / BN) * KSPLIT;
     }
+    (void)hipGetLastError();   /* the check after the launch reports it alone */
     hipLaunchKernelGGL(mlp_gemm_partials, dim3(g.first[L]), dim3(GEMM_THREADS), 0, m->stream, g);
     return hip_check(hipGetLastError(), "weight-gradient launch");
 }
@@ -713,6 +718,7 @@ int adam(bsim_mlp *m, int32_t batch, bool update, float lr, float *grad_out, flo
         ta.wfirst[l + 1] = ta.wfirst[l] + (ta.n[l] / 64) * (ta.k[l] / 64);
         ta.bfirst[l + 1] = ta.bfirst[l] + (ta.n[l] + ADAM_THREADS - 1) / ADAM_THREADS;
     }
+    (void)hipGetLastError();   /* the check after the launch reports it alone */
     hipLaunchKernelGGL(mlp_adam_tiles, dim3((uint32_t)(ta.wfirst[m->nlin] + ta.bfirst[m->nlin])),
                        dim3(ADAM_THREADS), 0, m->stream, ta);
     const hipError_t e = hipGetLastError();
@@ -734,6 +740,7 @@ int transpose_weights(bsim_mlp *m)
     }
     a.w = m->pbf;
     a.wt = m->pbf_t;
+    (void)hipGetLastError();   /* the check after the launch reports it alone */
     hipLaunchKernelGGL(mlp_transpose_w, dim3(a.first[m->nlin]), dim3(256), 0, m->stream, a);
     return hip_check(hipGetLastError(), "transpose launch");
 }
@@ -1036,6 +1043,7 @@ int bsim_mlp_gradients(bsim_mlp *m, const float *x, const float *y, int32_t batc
                                            hipMemcpyHostToDevice, m->stream), "hipMemcpyAsync");
     if (rc) return rc;
     const int64_t tot = (int64_t)batch * kin;
+    (void)hipGetLastError();   /* the check after the launch reports it alone */
     hipLaunchKernelGGL(mlp_convert_x, dim3((uint32_t)((tot + 255) / 256)), dim3(256), 0, m->stream,
                        m->xstage, m->act[0], m->act_t[0], (int64_t)batch, din, kin);
     rc = hip_check(hipGetLastError(), "convert launch");
@@ -1064,6 +1072,7 @@ int bsim_mlp_predict(bsim_mlp *m, const float *x, int64_t n, float *y)
                                                hipMemcpyHostToDevice, m->stream), "hipMemcpyAsync");
         if (rc) break;
         const int64_t tot = (int64_t)batch * kin;
+        (void)hipGetLastError();   /* the check after the launch reports it alone */
         hipLaunchKernelGGL(mlp_convert_x, dim3((uint32_t)((tot + 255) / 256)), dim3(256), 0,
                            m->stream, m->xstage, m->act[0], (uint16_t *)nullptr, (int64_t)batch,
                            din, kin);

@@ -917,6 +917,7 @@ __global__ __launch_bounds__(ROWS_THREADS) void bsim_rows_kernel(bsim_dev d)
 extern "C" hipError_t bsim_launch_rows(const bsim_dev *d, hipStream_t stream)
 {
     if (d->nsrc <= 0) return hipSuccess;
+    (void)hipGetLastError();       /* the check below then reports this launch alone */
     hipLaunchKernelGGL(bsim_rows_kernel, dim3(d->nsrc), dim3(ROWS_THREADS), 0, stream, *d);
     return hipGetLastError();
 }
@@ -933,6 +934,7 @@ extern "C" hipError_t bsim_launch_step(const bsim_dev *d, int32_t t, int32_t fla
     if (d->ntiles <= 0) return hipSuccess;
     const dim3 grid(d->nslots), block(NTHREAD);
     const size_t lds = bsim_step_lds_bytes(d->tile);
+    (void)hipGetLastError();       /* the check below then reports this launch alone */
     switch (d->tile / (NWAVE * WAVE)) {
     case 1: hipLaunchKernelGGL(bsim_step_kernel<1>, grid, block, lds, stream, *d, t, flags); break;
     case 2: hipLaunchKernelGGL(bsim_step_kernel<2>, grid, block, lds, stream, *d, t, flags); break;
