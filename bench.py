@@ -226,9 +226,19 @@ def bench_sweep(a):
     print(f"[bench] c3: {len(pts)} networks, {sum(p.n for p in pts)} neurons, ~{nsyn_est:.3g} "
           f"synapses, {steps} cycles each", file=sys.stderr, flush=True)
     t0 = time.time()
-    tile, tt = sweep.autotune_tile(pts[:a.c3_tune_nets], steps=100)
-    print(f"[bench] autotuned tile {tile} from {tt} ({time.time() - t0:.1f}s)", file=sys.stderr,
-          flush=True)
+    # the tuning batch: the first networks, up to --c3-tune-nets of them and
+    # --c3-tune-syn synapses (the whole sweep at the default sizes)
+    ntune, tsyn = 0, 0
+    for p in pts[:a.c3_tune_nets]:
+        ps = p.fan_in * sum(p.sizes[1:]) + p.drive_fan_in * p.sizes[0]
+        if ntune and tsyn + ps > a.c3_tune_syn:
+            break
+        ntune += 1
+        tsyn += ps
+    (tile, rank_every), tt = sweep.autotune(pts[:ntune])
+    print(f"[bench] autotune batch: {ntune} networks, ~{tsyn:.3g} synapses", file=sys.stderr, flush=True)
+    print(f"[bench] autotuned tile {tile}, tiles ranked every {rank_every} cycles (0: never), "
+          f"from {tt} ({time.time() - t0:.1f}s)", file=sys.stderr, flush=True)
     rng = np.random.default_rng(a.c3_seed + 1)
     sample = sorted(rng.choice(len(pts), size=min(a.c3_solo, len(pts)), replace=False).tolist())
     # batched: consecutive networks in simulations of at most --c3-sim-syn
@@ -250,7 +260,8 @@ def bench_sweep(a):
     secs_b, events, ntiles, nsyn = 0.0, 0, 0, 0
     for gi, grp in enumerate(groups):
         _, sb, info = sweep.run_batch_parts(grp, steps, tile=tile, record=False,
-                                            part_synapses=int(a.c3_batch_syn))
+                                            part_synapses=int(a.c3_batch_syn),
+                                            rank_every=rank_every)
         secs_b += sb
         events += info["events"]
         ntiles += info["ntiles"]
@@ -260,7 +271,8 @@ def bench_sweep(a):
     print(f"[bench] batched: {secs_b:.3f}s for {len(pts)} networks in {len(groups)} "
           f"simulation(s) ({ntiles} tiles, {nsyn} synapses, {events} events; built and run in "
           f"{time.time() - t0:.0f}s)", file=sys.stderr, flush=True)
-    rast, _, _ = sweep.run_batch([pts[i] for i in sample], steps, tile=tile, record=True)
+    rast, _, _ = sweep.run_batch([pts[i] for i in sample], steps, tile=tile, record=True,
+                                 rank_every=rank_every)
     secs_s, exact = 0.0, 0
     for j, i in enumerate(sample):
         _, dt_solo = sweep.run_solo(pts[i], steps, tile=tile, record=False)
@@ -290,7 +302,8 @@ def bench_sweep(a):
                                f"sizes log-uniform [{size[0]}, {size[1]}], fan-in U[100, 1000], "
                                f"w0 U[0.3, 1.5], inhibitory U[0.1, 0.4], drive U[5, 50] Hz, "
                                f"delays 1..U{{6..20}}, dt 0.1 ms, {steps} cycles",
-                   "tile": tile, "autotune": {str(k): v for k, v in tt.items()},
+                   "tile": tile, "rank_every": rank_every, "autotune_networks": ntune,
+                   "autotune": {f"tile {k[0]}, rank every {k[1]}": v for k, v in tt.items()},
                    "simulations": len(groups), "synapses": nsyn,
                    "parallelism": "single GPU"},
         "one_at_a_time_networks_per_s": nps_s,
@@ -460,7 +473,9 @@ def main():
     ap.add_argument("--last", type=int, default=0)
     ap.add_argument("--c3-nets", type=int, default=512)
     ap.add_argument("--c3-solo", type=int, default=32)
-    ap.add_argument("--c3-tune-nets", type=int, default=64)
+    ap.add_argument("--c3-tune-nets", type=int, default=512)
+    ap.add_argument("--c3-tune-syn", type=float, default=1.5e9,
+                    help="synapses of the autotuner's probe batch")
     ap.add_argument("--c3-steps", type=int, default=2000)
     ap.add_argument("--c3-min-size", type=int, default=300)
     ap.add_argument("--c3-max-size", type=int, default=3000)

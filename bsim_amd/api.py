@@ -60,6 +60,7 @@ _SIGS = {
     "bsim_sim_time_steps": (C.c_int32, [P, C.c_int32, P, P]),
     "bsim_sim_get_info": (C.c_int32, [P, P]),
     "bsim_sim_stream": (P, [P]),
+    "bsim_sim_set_schedule": (C.c_int32, [P, C.c_int32]),
     "bsim_sim_destroy": (None, [P]),
     "bsim_net_gen_standard": (C.c_int64, [P, P]),
     "bsim_net_kinds": (C.c_int32, [P, P]),
@@ -523,3 +524,10 @@ class Simulation:
 
     def stream(self):
         return lib().bsim_sim_stream(self._h)
+
+    def set_schedule(self, rank_every):
+        """Rank the tiles heaviest first every rank_every cycles (0: never);
+        results do not depend on it (bsim_sim_set_schedule)."""
+        r = lib().bsim_sim_set_schedule(self._h, int(rank_every))
+        if r < 0:
+            _err(r, "bsim_sim_set_schedule")

@@ -325,6 +325,12 @@ extern "C" {
 
 /* Checks the simulation config and makes its device current (0, or a
  * failure code). */
+/* Tile schedule: the row-list kernel ranks the tiles heaviest first every
+ * kRankEvery cycles (the volleys of the standard network move slowly against
+ * 4 cycles: every 4th cycle keeps the gain of ranking at a quarter of its
+ * cost); bsim_sim_set_schedule changes it. */
+static const int32_t kRankEvery = 4;
+
 static int32_t check_config(const bsim_sim_config &c, int32_t N, const char *who)
 {
     if (c.rec_steps < 0 || c.n_rec < 0 || (c.n_rec > 0 && !c.rec_ids))
@@ -541,6 +547,7 @@ static bsim_sim *sim_from_layout(bsim::Layout &L, const bsim_sim_config &c, bsim
             d.perm = dev_upload(sim, perm, e);
             if (e != hipSuccess) goto hip_fail;
         }
+        d.rank_every = kRankEvery;
         if (!(d.tile_load = dev_alloc<uint64_t>(sim, nt, e))) goto hip_fail;
         if ((e = hipMemsetAsync(d.tile_load, 0, nt * sizeof(uint64_t), sim->stream)) != hipSuccess) goto hip_fail;
         if (!(d.tile_cls = dev_alloc<uint8_t>(sim, nt, e))) goto hip_fail;
@@ -918,6 +925,29 @@ int32_t bsim_sim_get_info(const bsim_sim *sim, bsim_sim_info *info)
 }
 
 void *bsim_sim_stream(bsim_sim *sim) { return sim ? (void *)sim->stream : nullptr; }
+
+int32_t bsim_sim_set_schedule(bsim_sim *sim, int32_t rank_every)
+{
+    if (!sim || rank_every < 0) return fail(BSIM_E_ARG, "bsim_sim_set_schedule: bad argument");
+    if (int32_t rc = use_device(sim)) return rc;
+    if (sim->d.rank_every == rank_every) return 0;
+    hipError_t e;
+    /* the captured launches hold the kernel arguments: capture again; the
+     * loads of a tile ranked later start from zero */
+    if ((e = hipStreamSynchronize(sim->stream)) != hipSuccess ||
+        (e = hipMemsetAsync(sim->d.tile_load, 0, (size_t)(sim->d.ntiles > 0 ? sim->d.ntiles : 1) * 8,
+                            sim->stream)) != hipSuccess ||
+        (e = hipStreamSynchronize(sim->stream)) != hipSuccess)
+        return fail(BSIM_E_HIP, std::string("bsim_sim_set_schedule: ") + hipGetErrorString(e));
+    if (sim->gexec) {
+        if ((e = hipGraphExecDestroy(sim->gexec)) != hipSuccess || (e = hipGraphDestroy(sim->graph)) != hipSuccess)
+            return fail(BSIM_E_HIP, std::string("bsim_sim_set_schedule: ") + hipGetErrorString(e));
+        sim->gexec = nullptr;
+        sim->graph = nullptr;
+    }
+    sim->d.rank_every = rank_every;
+    return 0;
+}
 
 void bsim_sim_destroy(bsim_sim *sim) { sim_free(sim); }
 

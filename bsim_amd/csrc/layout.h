@@ -145,6 +145,9 @@ typedef struct bsim_dev {
                                    row-list workgroups finished */
     int32_t *perm;              /* [nslots] tile of each step workgroup (-1: none) */
     uint64_t *tile_load;        /* [ntiles] expected work of the coming cycle (x 16) */
+    int32_t rank_every;         /* tiles ranked heaviest first every this many cycles (0: never,
+                                   the tiles stay in their initial order) */
+    int32_t pad5;
     uint8_t *tile_cls;          /* [ntiles] its half-octave class */
     const int64_t *chunk_tiles_off;  /* [nsrc + 1] tiles that read each chunk: */
     const int32_t *chunk_tiles;      /*   tile ids */

@@ -64,9 +64,6 @@
 #define MAXR 128                      /* source ranges per batch */
 #define ROWS_THREADS 1024             /* row-list kernel: 4 pres per thread */
 #define HEAVY_WGS 256                 /* workgroups run at raised wave priority */
-#ifndef RANK_EVERY
-#define RANK_EVERY 4                  /* cycles between tile rankings */
-#endif
 static_assert(BSIM_CHUNK == 4 * ROWS_THREADS, "a source chunk is one row-list workgroup");
 
 #ifndef MARK_MIN_S
@@ -883,10 +880,10 @@ __global__ __launch_bounds__(ROWS_THREADS) void bsim_rows_kernel(bsim_dev d)
         const uint32_t rb = (uint32_t)(p0 + x - lo) * (uint32_t)D;
         for (uint64_t mm = m[x]; mm; mm &= mm - 1) out[pos++] = rb + (uint32_t)__builtin_ctzll(mm);
     }
-    /* the tiles are ranked every RANK_EVERY cycles (the order only affects
+    /* the tiles are ranked every d.rank_every cycles (the order only affects
      * speed); d.clock[0], the next step launch's cycle, is the same for every
      * workgroup of this launch */
-    const bool rank = RANK_EVERY == 1 || d.clock[0] % RANK_EVERY == 0;
+    const bool rank = d.rank_every == 1 || (d.rank_every > 1 && d.clock[0] % d.rank_every == 0);
     /* for every tile that reads this chunk: the list's length in the tile's
      * range entry and, when ranking, the tile's expected work -- rows
      * weighted by the tile's mean deliveries per row plus a fixed cost of

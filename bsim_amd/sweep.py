@@ -159,11 +159,14 @@ def split_raster(counts, ids, first, sizes):
     return res
 
 
-def run_batch(points, steps, tile=0, record=True):
-    """Simulates the points together; returns (per-network rasters or None,
-    seconds of the stepping, simulation info)."""
+def run_batch(points, steps, tile=0, record=True, rank_every=None):
+    """Simulates the points together (with the tile schedule rank_every, or
+    the default); returns (per-network rasters or None, seconds of the
+    stepping, simulation info)."""
     b = build(points)
     sim = api.Simulation(b.net, netspec.DT, tile=tile, rec_steps=steps if record else 0)
+    if rank_every is not None:
+        sim.set_schedule(rank_every)
     t0 = time.perf_counter()
     sim.step(steps)
     sim.finish()
@@ -179,7 +182,7 @@ def run_batch(points, steps, tile=0, record=True):
     return rasters, secs, dict(info, events=ev)
 
 
-def run_batch_parts(points, steps, tile=0, record=True, part_synapses=int(1.5e9)):
+def run_batch_parts(points, steps, tile=0, record=True, part_synapses=int(1.5e9), rank_every=None):
     """run_batch for sweeps too large for one host build: the networks are
     generated, laid out and uploaded in parts of at most part_synapses
     synapses and joined on the device into one simulation
@@ -188,6 +191,8 @@ def run_batch_parts(points, steps, tile=0, record=True, part_synapses=int(1.5e9)
     sim, first = api.Simulation.from_chains([p.kwargs() for p in points], netspec.DT, tile=tile,
                                             rec_steps=steps if record else 0,
                                             part_synapses=part_synapses)
+    if rank_every is not None:
+        sim.set_schedule(rank_every)
     t0 = time.perf_counter()
     sim.step(steps)
     sim.finish()
@@ -218,18 +223,33 @@ def run_solo(point, steps, tile=0, record=True):
     return raster, secs
 
 
-def autotune_tile(points, steps=100, tiles=(192, 384, 768)):
-    """The project's tile autotuner: the tile size with the fastest short run
-    of the batch (no recording).  Returns (best tile, {tile: seconds})."""
+def autotune(points, steps=128, preroll=256, tiles=(192, 384, 768), ranks=(0, 1, 4)):
+    """The project's schedule autotuner for a batch: the tile size (targets
+    per workgroup) and the tile schedule (tiles ranked heaviest first every
+    r cycles, 0 = never, Simulation.set_schedule) with the fastest run of
+    `steps` cycles of the batch, timed after `preroll` cycles so that the
+    probe sees the swept regime rather than the cold start (no recording;
+    results do not depend on either choice).  Returns ((tile, rank_every),
+    {(tile, rank_every): seconds})."""
     times = {}
     b = build(points)
     for t in tiles:
         sim = api.Simulation(b.net, netspec.DT, tile=t)
-        sim.step(min(64, steps))          # warm-up: first launches, graph capture
+        sim.step(preroll)
         sim.sync()
-        t0 = time.perf_counter()
-        sim.step(steps)
-        sim.sync()
-        times[t] = time.perf_counter() - t0
+        for r in ranks:
+            sim.set_schedule(r)
+            sim.step(64)                      # graph capture, first ranking
+            sim.sync()
+            t0 = time.perf_counter()
+            sim.step(steps)
+            sim.sync()
+            times[(t, r)] = time.perf_counter() - t0
         sim.close()
     return min(times, key=times.get), times
+
+
+def autotune_tile(points, steps=128, preroll=256, tiles=(192, 384, 768)):
+    """The tile size alone (the default schedule): (best tile, {tile: seconds})."""
+    best, times = autotune(points, steps, preroll, tiles, ranks=(4,))
+    return best[0], {t: s for (t, _), s in times.items()}

@@ -214,6 +214,15 @@ int32_t bsim_sim_get_info(const bsim_sim *sim, bsim_sim_info *info);
  * work around the step. */
 void *bsim_sim_stream(bsim_sim *sim);
 
+/* Tile schedule (no counterpart in the reference, whose per-type launches
+ * take the neurons and synapses in id order with a fixed grid,
+ * SingleGPUSimulator.cu:100, 165-171): the order in which the
+ * step's workgroups take the tiles.  Every rank_every cycles the row-list
+ * kernel ranks the tiles by their expected deliveries, heaviest first (the
+ * default, 4); 0 keeps the tiles in their initial order.  Results do not
+ * depend on it.  Takes effect from the next step. */
+int32_t bsim_sim_set_schedule(bsim_sim *sim, int32_t rank_every);
+
 void bsim_sim_destroy(bsim_sim *sim);
 
 /* ---- workload generation and export (host) -------------------------------- */

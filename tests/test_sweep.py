@@ -98,6 +98,45 @@ def test_batch_rasters_equal_solo_rasters(tile):
 
 
 @pytest.mark.gpu
+def test_tile_schedule_does_not_change_results():
+    """The tile schedule (tiles ranked heaviest first every r cycles, or
+    never) only reorders the workgroups: every network's raster and the
+    deliveries are the same for r = 0, 1, 3 and the default, and switching
+    the schedule between calls continues the run bit for bit."""
+    steps = 300
+    pts = sweep.sample(10, seed=12, **SMALL)
+    ref, _, info = sweep.run_batch(pts, steps, tile=192)
+    for r in (0, 1, 3):
+        got, _, inf = sweep.run_batch(pts, steps, tile=192, rank_every=r)
+        assert inf["events"] == info["events"], r
+        for k in range(len(pts)):
+            for c in range(steps):
+                np.testing.assert_array_equal(got[k][c], ref[k][c], err_msg=f"r={r} network {k} cycle {c}")
+    b = sweep.build(pts)
+    sim = api.Simulation(b.net, netspec.DT, tile=192, rec_steps=steps)
+    for r in (0, 5, 1, 0, 4):
+        sim.set_schedule(r)
+        sim.step(60)
+    sim.finish()
+    sim.sync()
+    counts, ids, _ = sim.raster()
+    got = sweep.split_raster(counts, ids, b.first, b.sizes)
+    assert sim.events() == info["events"]
+    sim.close()
+    for k in range(len(pts)):
+        for c in range(steps):
+            np.testing.assert_array_equal(got[k][c], ref[k][c], err_msg=f"switched, network {k} cycle {c}")
+
+
+@pytest.mark.gpu
+def test_autotune_times_every_candidate():
+    pts = sweep.sample(6, seed=13, **SMALL)
+    (tile, r), times = sweep.autotune(pts, steps=64, preroll=64, tiles=(192, 384), ranks=(0, 4))
+    assert set(times) == {(t, x) for t in (192, 384) for x in (0, 4)}
+    assert (tile, r) in times and times[(tile, r)] == min(times.values())
+
+
+@pytest.mark.gpu
 def test_sweep_network_alone_matches_oracle():
     steps = 250
     for p in sweep.sample(2, seed=13, **SMALL):
