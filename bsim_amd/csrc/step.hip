@@ -104,6 +104,21 @@ __device__ __forceinline__ uint32_t readlane(uint32_t x, int32_t l)
     return __builtin_amdgcn_readlane(x, l);
 }
 
+/* A value that is the same in every lane (an LDS word all lanes read, the
+ * wavefront index), moved to a scalar register: branches and loops on it are
+ * then scalar branches, not exec-masked ones.  Without this the compiler
+ * cannot tell that such a loop exits at once for the whole wavefront; it
+ * builds the loop with lane masks, merges its exit paths, and waits for all
+ * loads in flight where the paths join. */
+__device__ __forceinline__ uint32_t uniform(uint32_t x) { return __builtin_amdgcn_readfirstlane(x); }
+
+__device__ __forceinline__ uint32_t wave_index() { return uniform(threadIdx.x / WAVE); }
+
+/* s_waitcnt vmcnt(0) (GFX9 encoding: vmcnt 0, expcnt and lgkmcnt at their
+ * maxima, i.e. not waited for): this wavefront's vector memory loads have
+ * completed */
+__device__ __forceinline__ void wait_all_loads() { __builtin_amdgcn_s_waitcnt(0x0F70); }
+
 /* orders this wavefront's LDS accesses (lanes exchange data through LDS) */
 __device__ __forceinline__ void wave_sync()
 {
@@ -162,7 +177,7 @@ __device__ __forceinline__ void load_win(const uint32_t *rows, RowWin<R> &W, int
                                          int32_t nrows)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
-    const uint32_t q = threadIdx.x / WAVE;
+    const uint32_t q = wave_index();
     uint32_t s0[R], sum = 0;
 #pragma unroll
     for (int r = 0; r < R; r++) {
@@ -221,7 +236,7 @@ __device__ __forceinline__ void load_step(const Shared &S, const uint32_t *rows,
                                           const bsim_syn *syn, Step &st)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
-    const uint32_t q = threadIdx.x / WAVE;
+    const uint32_t q = wave_index();
     uint32_t *al = S.alist + q * SSTEP;
     uint64_t *pm = S.pmask + q * NGRP;
     const uint32_t e = min(b + SSTEP, tot);
@@ -355,7 +370,7 @@ template <int S_>
 __device__ __forceinline__ void apply_step(const Shared &S, const Step &st, uint32_t qbase, Acc<S_> &acc)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
-    const uint32_t q = threadIdx.x / WAVE;
+    const uint32_t q = wave_index();
     uint2 *stg = S.stage + q * SSTEP;
     uint64_t *om = S.omask + q * (NGRP * WAVE);
     const uint64_t mybit = 1ull << lane;
@@ -393,7 +408,12 @@ __device__ __forceinline__ void apply_step(const Shared &S, const Step &st, uint
 }
 
 /* Streams this wavefront's quarter of the staged rows [0, nrows).  The loads
- * of step s+1 are in flight while step s is applied. */
+ * of step s+1 are in flight while step s is applied.  No load is issued past
+ * the last step: a load left in flight at the end of the stream would still be
+ * pending when the next window starts (the workgroup barrier waits for LDS
+ * only), and its destination registers, reused there, would make the compiler
+ * wait for every load in flight at the top of each step -- the step pipeline
+ * would then hold one step's loads at a time. */
 template <int S_>
 __device__ void stream_rows(const Shared &S, const uint32_t *rows, int32_t nrows, uint32_t tot,
                             uint32_t qbase, uint32_t nsyn, const bsim_syn *syn,
@@ -407,14 +427,20 @@ __device__ void stream_rows(const Shared &S, const uint32_t *rows, int32_t nrows
     load_step(S, rows, W, 0, tot, nrows, nsyn, syn, A);
     uint32_t b = 0;
     while (true) {
+        if (b + SSTEP >= tot) {                               /* uniform: A is the last step */
+            apply_step<S_>(S, A, qbase, acc);
+            break;
+        }
         load_step(S, rows, W, b + SSTEP, tot, nrows, nsyn, syn, B);
         apply_step<S_>(S, A, qbase, acc);
         b += SSTEP;
-        if (b >= tot) break;
+        if (b + SSTEP >= tot) {                               /* uniform: B is the last step */
+            apply_step<S_>(S, B, qbase, acc);
+            break;
+        }
         load_step(S, rows, W, b + SSTEP, tot, nrows, nsyn, syn, A);
         apply_step<S_>(S, B, qbase, acc);
         b += SSTEP;
-        if (b >= tot) break;
     }
 }
 
@@ -453,7 +479,11 @@ __device__ __forceinline__ void prod_entries(const Shared &S, const bsim_dev &d,
  * its rows (from the list entries E) -- the part starts and the end of the
  * tile's sub-rows, one aligned 16-byte load -- and the stream length of each
  * part.  The rows of a lane ascend, so their source range is found by walking
- * forward. */
+ * forward.  Every lane uses all of its E and Tb registers, including those of
+ * rows past the window's end (zeros; the slots of the buffer they fill are
+ * never read), so that no load of the producer is still pending, as far as
+ * the compiler can tell, when the registers are reused by the streaming code
+ * that shares the program. */
 __device__ __forceinline__ void prod_stage(const Shared &S, const bsim_dev &d, int32_t buf, int32_t win,
                                            uint32_t nrows_tot, int32_t nrb, const uint32_t (&E)[RPL])
 {
@@ -464,12 +494,10 @@ __device__ __forceinline__ void prod_stage(const Shared &S, const bsim_dev &d, i
 #pragma unroll
     for (int x = 0; x < RPL; x++) {
         const uint32_t j = (uint32_t)win * RCAP + lane + WAVE * x;
+        while (ri + 1 < nrb && S.cum[ri + 1] <= j) ri++;      /* ri < nrb */
+        const int64_t e0 = S.obase[ri] + (int64_t)E[x] * S.stride[ri];
         Tb[x] = make_uint4(0u, 0u, 0u, 0u);
-        if (j < nrows_tot) {
-            while (ri + 1 < nrb && S.cum[ri + 1] <= j) ri++;
-            const int64_t e0 = S.obase[ri] + (int64_t)E[x] * S.stride[ri];
-            Tb[x] = *reinterpret_cast<const uint4 *>(d.off_tab + e0);
-        }
+        if (j < nrows_tot) Tb[x] = *reinterpret_cast<const uint4 *>(d.off_tab + e0);
     }
     uint32_t ql[NWAVE];
 #pragma unroll
@@ -477,12 +505,10 @@ __device__ __forceinline__ void prod_stage(const Shared &S, const bsim_dev &d, i
 #pragma unroll
     for (int x = 0; x < RPL; x++) {
         const uint32_t jj = lane + WAVE * x;
-        if ((uint32_t)win * RCAP + jj < nrows_tot) {
-            rows[jj] = Tb[x];
-            ql[0] += Tb[x].y - Tb[x].x;
-            ql[1] += Tb[x].z - Tb[x].y;
-            ql[2] += Tb[x].w - Tb[x].z;
-        }
+        rows[jj] = Tb[x];
+        ql[0] += Tb[x].y - Tb[x].x;
+        ql[1] += Tb[x].z - Tb[x].y;
+        ql[2] += Tb[x].w - Tb[x].z;
     }
 #pragma unroll
     for (int q = 0; q < NWAVE; q++) {
@@ -500,7 +526,7 @@ __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, in
                               int64_t &events, Acc<S_> &acc)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
-    const uint32_t w = threadIdx.x / WAVE;
+    const uint32_t w = wave_index();
     const bool prod = w == NWAVE;
     if (prod) {
         /* range metadata and the cumulative row counts of the batch */
@@ -531,26 +557,37 @@ __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, in
         if (lane == WAVE - 1) S.cum[nrb] = incl;
     }
     __syncthreads();
-    const uint32_t nrows_tot = S.cum[nrb];
+    const uint32_t nrows_tot = uniform(S.cum[nrb]);
     const int32_t nwin = (int32_t)((nrows_tot + RCAP - 1) / RCAP);
     if (nwin == 0) return;                                    /* uniform */
-    uint32_t E[RPL];
+    /* The producer issues the list entries of the window after next (En)
+     * before it stages the next one (from E), and ends each turn with all of
+     * its loads complete: the staging waits for its row-table loads, issued
+     * after those entries, so the wait costs nothing, and the compiler, which
+     * sees one program for all wavefronts, then knows that no load of the
+     * producer is pending in the streaming code (otherwise it makes the
+     * streaming wavefronts wait for loads that only the producer issued). */
+    uint32_t E[RPL], En[RPL];
     if (prod) {
         prod_entries(S, d, 0, nrows_tot, nrb, E);
+        if (nwin > 1) prod_entries(S, d, 1, nrows_tot, nrb, En);
         prod_stage(S, d, 0, 0, nrows_tot, nrb, E);
-        if (nwin > 1) prod_entries(S, d, 1, nrows_tot, nrb, E);
+        wait_all_loads();
     }
     __syncthreads();
     for (int32_t i = 0; i < nwin; i++) {
         const int32_t buf = i & 1;
         if (prod) {
             if (i + 1 < nwin) {
+#pragma unroll
+                for (int x = 0; x < RPL; x++) E[x] = En[x];
+                if (i + 2 < nwin) prod_entries(S, d, i + 2, nrows_tot, nrb, En);
                 prod_stage(S, d, buf ^ 1, i + 1, nrows_tot, nrb, E);
-                if (i + 2 < nwin) prod_entries(S, d, i + 2, nrows_tot, nrb, E);
             }
+            wait_all_loads();
         } else {
             const int32_t nrows = (int32_t)min((uint32_t)RCAP, nrows_tot - (uint32_t)i * RCAP);
-            const uint32_t qtot = S.tot[buf * NWAVE + w];
+            const uint32_t qtot = uniform(S.tot[buf * NWAVE + w]);
             events += qtot;
             stream_rows<S_>(S, S.rows + buf * (RCAP * ROWW), nrows, qtot, qbase, nsyn, syn, acc);
         }
@@ -575,7 +612,7 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
     const int32_t k = d.perm[blockIdx.x];       /* -1: an empty slot */
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & (WAVE - 1);
-    const uint32_t w = tid / WAVE;
+    const uint32_t w = wave_index();
     const uint32_t qbase = w * ((uint32_t)T / NWAVE);
 
     Shared S;
