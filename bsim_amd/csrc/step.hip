@@ -447,10 +447,11 @@ __device__ void stream_rows(const Shared &S, const uint32_t *rows, int32_t nrows
 /* ---- the producer wavefront: staging of row windows ---------------------- */
 
 /* Row j of a window of the current range batch belongs to the source range
- * whose cumulative row interval holds it. */
+ * whose cumulative row interval holds it: the largest i with cum[i] <= j
+ * (ranges without active rows repeat the count and are passed over). */
 __device__ __forceinline__ int32_t range_of(const Shared &S, uint32_t j, int32_t nrb)
 {
-    int32_t a = 0, b = nrb - 1;           /* largest i with cum[i] <= j */
+    int32_t a = 0, b = nrb - 1;
     while (a < b) {
         const int32_t m = (a + b + 1) >> 1;
         if (S.cum[m] <= j) a = m; else b = m - 1;
@@ -458,12 +459,27 @@ __device__ __forceinline__ int32_t range_of(const Shared &S, uint32_t j, int32_t
     return a;
 }
 
-/* Active-row list entries of window `win` (lane holds rows lane + 64 x). */
+/* The source range index of each of a lane's rows (< MAXR = 128) is found
+ * once per window, when its list entries are read, and kept packed four to a
+ * register (RW words) until the window is staged. */
+static_assert(MAXR <= 256, "range indices are packed in bytes");
+#define RW ((RPL + 3) / 4)
+__device__ __forceinline__ int32_t range_at(const uint32_t (&R)[RW], int x)
+{
+    return (int32_t)((R[x / 4] >> (8 * (x % 4))) & 0xFFu);
+}
+
+/* Active-row list entries of window `win` (lane holds rows lane + 64 x) and
+ * the source range of each row (R).  The rows of a lane ascend, so their
+ * ranges are found by walking forward from the first one's. */
 __device__ __forceinline__ void prod_entries(const Shared &S, const bsim_dev &d, int32_t win,
-                                             uint32_t nrows_tot, int32_t nrb, uint32_t (&E)[RPL])
+                                             uint32_t nrows_tot, int32_t nrb, uint32_t (&E)[RPL],
+                                             uint32_t (&R)[RW])
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     int32_t ri = range_of(S, (uint32_t)win * RCAP + lane, nrb);
+#pragma unroll
+    for (int w = 0; w < RW; w++) R[w] = 0u;
 #pragma unroll
     for (int x = 0; x < RPL; x++) {
         const uint32_t j = (uint32_t)win * RCAP + lane + WAVE * x;
@@ -472,6 +488,7 @@ __device__ __forceinline__ void prod_entries(const Shared &S, const bsim_dev &d,
             while (ri + 1 < nrb && S.cum[ri + 1] <= j) ri++;
             E[x] = d.src_list[S.lbase[ri] + (j - S.cum[ri])];
         }
+        R[x / 4] |= (uint32_t)ri << (8 * (x % 4));
     }
 }
 
@@ -485,16 +502,16 @@ __device__ __forceinline__ void prod_entries(const Shared &S, const bsim_dev &d,
  * the compiler can tell, when the registers are reused by the streaming code
  * that shares the program. */
 __device__ __forceinline__ void prod_stage(const Shared &S, const bsim_dev &d, int32_t buf, int32_t win,
-                                           uint32_t nrows_tot, int32_t nrb, const uint32_t (&E)[RPL])
+                                           uint32_t nrows_tot, const uint32_t (&E)[RPL],
+                                           const uint32_t (&R)[RW])
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     uint4 *rows = reinterpret_cast<uint4 *>(S.rows + buf * (RCAP * ROWW));
     uint4 Tb[RPL];
-    int32_t ri = range_of(S, (uint32_t)win * RCAP + lane, nrb);
 #pragma unroll
     for (int x = 0; x < RPL; x++) {
         const uint32_t j = (uint32_t)win * RCAP + lane + WAVE * x;
-        while (ri + 1 < nrb && S.cum[ri + 1] <= j) ri++;      /* ri < nrb */
+        const int32_t ri = range_at(R, x);
         const int64_t e0 = S.obase[ri] + (int64_t)E[x] * S.stride[ri];
         Tb[x] = make_uint4(0u, 0u, 0u, 0u);
         if (j < nrows_tot) Tb[x] = *reinterpret_cast<const uint4 *>(d.off_tab + e0);
@@ -568,10 +585,11 @@ __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, in
      * producer is pending in the streaming code (otherwise it makes the
      * streaming wavefronts wait for loads that only the producer issued). */
     uint32_t E[RPL], En[RPL];
+    uint32_t R[RW], Rn[RW];
     if (prod) {
-        prod_entries(S, d, 0, nrows_tot, nrb, E);
-        if (nwin > 1) prod_entries(S, d, 1, nrows_tot, nrb, En);
-        prod_stage(S, d, 0, 0, nrows_tot, nrb, E);
+        prod_entries(S, d, 0, nrows_tot, nrb, E, R);
+        if (nwin > 1) prod_entries(S, d, 1, nrows_tot, nrb, En, Rn);
+        prod_stage(S, d, 0, 0, nrows_tot, E, R);
         wait_all_loads();
     }
     __syncthreads();
@@ -581,8 +599,10 @@ __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, in
             if (i + 1 < nwin) {
 #pragma unroll
                 for (int x = 0; x < RPL; x++) E[x] = En[x];
-                if (i + 2 < nwin) prod_entries(S, d, i + 2, nrows_tot, nrb, En);
-                prod_stage(S, d, buf ^ 1, i + 1, nrows_tot, nrb, E);
+#pragma unroll
+                for (int x = 0; x < RW; x++) R[x] = Rn[x];
+                if (i + 2 < nwin) prod_entries(S, d, i + 2, nrows_tot, nrb, En, Rn);
+                prod_stage(S, d, buf ^ 1, i + 1, nrows_tot, E, R);
             }
             wait_all_loads();
         } else {
