@@ -470,14 +470,16 @@ __device__ __forceinline__ int32_t range_at(const uint32_t (&R)[RW], int x)
 }
 
 /* Active-row list entries of window `win` (lane holds rows lane + 64 x) and
- * the source range of each row (R).  The rows of a lane ascend, so their
- * ranges are found by walking forward from the first one's. */
+ * the source range of each row (R).  The rows of a lane ascend from window
+ * to window, so their ranges are found by walking forward from the range of
+ * the lane's last row in the previous window (ri, carried by the caller; a
+ * search for the first window). */
 __device__ __forceinline__ void prod_entries(const Shared &S, const bsim_dev &d, int32_t win,
                                              uint32_t nrows_tot, int32_t nrb, uint32_t (&E)[RPL],
-                                             uint32_t (&R)[RW])
+                                             uint32_t (&R)[RW], int32_t &ri)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
-    int32_t ri = range_of(S, (uint32_t)win * RCAP + lane, nrb);
+    if (win == 0) ri = range_of(S, lane, nrb);
 #pragma unroll
     for (int w = 0; w < RW; w++) R[w] = 0u;
 #pragma unroll
@@ -488,7 +490,7 @@ __device__ __forceinline__ void prod_entries(const Shared &S, const bsim_dev &d,
             while (ri + 1 < nrb && S.cum[ri + 1] <= j) ri++;
             E[x] = d.src_list[S.lbase[ri] + (j - S.cum[ri])];
         }
-        R[x / 4] |= (uint32_t)ri << (8 * (x % 4));
+        R[x / 4] |= (uint32_t)ri << (8 * (x % 4));      /* ri < nrb <= MAXR */
     }
 }
 
@@ -586,9 +588,10 @@ __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, in
      * streaming wavefronts wait for loads that only the producer issued). */
     uint32_t E[RPL], En[RPL];
     uint32_t R[RW], Rn[RW];
+    int32_t ri = 0;
     if (prod) {
-        prod_entries(S, d, 0, nrows_tot, nrb, E, R);
-        if (nwin > 1) prod_entries(S, d, 1, nrows_tot, nrb, En, Rn);
+        prod_entries(S, d, 0, nrows_tot, nrb, E, R, ri);
+        if (nwin > 1) prod_entries(S, d, 1, nrows_tot, nrb, En, Rn, ri);
         prod_stage(S, d, 0, 0, nrows_tot, E, R);
         wait_all_loads();
     }
@@ -601,7 +604,7 @@ __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, in
                 for (int x = 0; x < RPL; x++) E[x] = En[x];
 #pragma unroll
                 for (int x = 0; x < RW; x++) R[x] = Rn[x];
-                if (i + 2 < nwin) prod_entries(S, d, i + 2, nrows_tot, nrb, En, Rn);
+                if (i + 2 < nwin) prod_entries(S, d, i + 2, nrows_tot, nrb, En, Rn, ri);
                 prod_stage(S, d, buf ^ 1, i + 1, nrows_tot, E, R);
             }
             wait_all_loads();
