@@ -56,8 +56,6 @@
 #define WAVE 64
 #define NWAVE BSIM_NWAVE              /* streaming wavefronts = target quarters */
 #define NTHREAD (WAVE * (NWAVE + 1))  /* plus one producer wavefront */
-#define RCAP 384                      /* rows per staged window */
-#define RPL (RCAP / WAVE)             /* rows per producer lane */
 #define ROWW (NWAVE + 1)              /* uint32 per staged row: part starts and end */
 #define NGRP 4                        /* 64-position groups per streaming step */
 #define SSTEP (NGRP * WAVE)
@@ -71,9 +69,24 @@ static_assert(BSIM_CHUNK == 4 * ROWS_THREADS, "a source chunk is one row-list wo
 #endif
 static_assert(ROWW == 4 && NWAVE == 3, "a staged row is one 16-byte vector of 3 part starts and the end");
 
+/* Rows per staged window (RCAP): 384 at one accumulator slot per lane (TILE
+ * 192, where LDS limits a CU to five workgroups), 512 from two slots up (four
+ * workgroups per CU either way, bounded by registers), so that a heavy tile
+ * passes fewer window barriers.  RPL rows per producer lane, RW words of
+ * packed range indices. */
+template <int S_>
+struct Win {
+    static constexpr int RCAP = S_ >= 2 ? 512 : 384;
+    static constexpr int RPL = RCAP / WAVE;
+    static constexpr int RW = (RPL + 3) / 4;
+};
+
 /* LDS bytes of a workgroup: lbase, obase | stage | rows | tot | alist | cum | stride | omask | pmask */
-#define LDS_FIXED (16 * MAXR + 8 * NWAVE * SSTEP + 4 * (2 * RCAP * ROWW + 2 * NWAVE + NWAVE * SSTEP) + \
-                   4 * (MAXR + 2) + 4 * MAXR + 8 * NWAVE * NGRP * WAVE + 8 * NWAVE * NGRP)
+__host__ __device__ constexpr size_t lds_bytes(int rcap)
+{
+    return 16 * MAXR + 8 * NWAVE * SSTEP + 4 * (2 * rcap * ROWW + 2 * NWAVE + NWAVE * SSTEP) +
+           4 * (MAXR + 2) + 4 * MAXR + 8 * NWAVE * NGRP * WAVE + 8 * NWAVE * NGRP;
+}
 
 __device__ __forceinline__ uint32_t bern_hash(uint64_t seed, uint32_t id, uint32_t t)
 {
@@ -463,7 +476,7 @@ __device__ __forceinline__ int32_t range_of(const Shared &S, uint32_t j, int32_t
  * once per window, when its list entries are read, and kept packed four to a
  * register (RW words) until the window is staged. */
 static_assert(MAXR <= 256, "range indices are packed in bytes");
-#define RW ((RPL + 3) / 4)
+template <int RW>
 __device__ __forceinline__ int32_t range_at(const uint32_t (&R)[RW], int x)
 {
     return (int32_t)((R[x / 4] >> (8 * (x % 4))) & 0xFFu);
@@ -474,6 +487,7 @@ __device__ __forceinline__ int32_t range_at(const uint32_t (&R)[RW], int x)
  * to window, so their ranges are found by walking forward from the range of
  * the lane's last row in the previous window (ri, carried by the caller; a
  * search for the first window). */
+template <int RCAP, int RPL = RCAP / WAVE, int RW = (RPL + 3) / 4>
 __device__ __forceinline__ void prod_entries(const Shared &S, const bsim_dev &d, int32_t win,
                                              uint32_t nrows_tot, int32_t nrb, uint32_t (&E)[RPL],
                                              uint32_t (&R)[RW], int32_t &ri)
@@ -503,6 +517,7 @@ __device__ __forceinline__ void prod_entries(const Shared &S, const bsim_dev &d,
  * never read), so that no load of the producer is still pending, as far as
  * the compiler can tell, when the registers are reused by the streaming code
  * that shares the program. */
+template <int RCAP, int RPL = RCAP / WAVE, int RW = (RPL + 3) / 4>
 __device__ __forceinline__ void prod_stage(const Shared &S, const bsim_dev &d, int32_t buf, int32_t win,
                                            uint32_t nrows_tot, const uint32_t (&E)[RPL],
                                            const uint32_t (&R)[RW])
@@ -513,7 +528,7 @@ __device__ __forceinline__ void prod_stage(const Shared &S, const bsim_dev &d, i
 #pragma unroll
     for (int x = 0; x < RPL; x++) {
         const uint32_t j = (uint32_t)win * RCAP + lane + WAVE * x;
-        const int32_t ri = range_at(R, x);
+        const int32_t ri = range_at<RW>(R, x);
         const int64_t e0 = S.obase[ri] + (int64_t)E[x] * S.stride[ri];
         Tb[x] = make_uint4(0u, 0u, 0u, 0u);
         if (j < nrows_tot) Tb[x] = *reinterpret_cast<const uint4 *>(d.off_tab + e0);
@@ -576,6 +591,7 @@ __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, in
         if (lane == WAVE - 1) S.cum[nrb] = incl;
     }
     __syncthreads();
+    constexpr int RCAP = Win<S_>::RCAP, RPL = Win<S_>::RPL, RW = Win<S_>::RW;
     const uint32_t nrows_tot = uniform(S.cum[nrb]);
     const int32_t nwin = (int32_t)((nrows_tot + RCAP - 1) / RCAP);
     if (nwin == 0) return;                                    /* uniform */
@@ -590,9 +606,9 @@ __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, in
     uint32_t R[RW], Rn[RW];
     int32_t ri = 0;
     if (prod) {
-        prod_entries(S, d, 0, nrows_tot, nrb, E, R, ri);
-        if (nwin > 1) prod_entries(S, d, 1, nrows_tot, nrb, En, Rn, ri);
-        prod_stage(S, d, 0, 0, nrows_tot, E, R);
+        prod_entries<RCAP>(S, d, 0, nrows_tot, nrb, E, R, ri);
+        if (nwin > 1) prod_entries<RCAP>(S, d, 1, nrows_tot, nrb, En, Rn, ri);
+        prod_stage<RCAP>(S, d, 0, 0, nrows_tot, E, R);
         wait_all_loads();
     }
     __syncthreads();
@@ -604,8 +620,8 @@ __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, in
                 for (int x = 0; x < RPL; x++) E[x] = En[x];
 #pragma unroll
                 for (int x = 0; x < RW; x++) R[x] = Rn[x];
-                if (i + 2 < nwin) prod_entries(S, d, i + 2, nrows_tot, nrb, En, Rn, ri);
-                prod_stage(S, d, buf ^ 1, i + 1, nrows_tot, E, R);
+                if (i + 2 < nwin) prod_entries<RCAP>(S, d, i + 2, nrows_tot, nrb, En, Rn, ri);
+                prod_stage<RCAP>(S, d, buf ^ 1, i + 1, nrows_tot, E, R);
             }
             wait_all_loads();
         } else {
@@ -645,12 +661,12 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
         S.obase = p8;                                 p8 += MAXR;
         S.stage = reinterpret_cast<uint2 *>(p8);
         uint32_t *p = reinterpret_cast<uint32_t *>(p8) + NWAVE * SSTEP * 2;
-        S.rows = p;                                   p += 2 * RCAP * ROWW;
+        S.rows = p;                                   p += 2 * Win<S_>::RCAP * ROWW;
         S.tot = p;                                    p += 2 * NWAVE;
         S.alist = p;                                  p += NWAVE * SSTEP;
         S.cum = p;                                    p += MAXR + 2;
         S.stride = p;                                 p += MAXR;
-        S.omask = reinterpret_cast<uint64_t *>(p);    /* 8-byte aligned: see LDS_FIXED */
+        S.omask = reinterpret_cast<uint64_t *>(p);    /* 8-byte aligned: see lds_bytes */
         S.pmask = S.omask + NWAVE * NGRP * WAVE;
     }
 
@@ -981,8 +997,7 @@ extern "C" hipError_t bsim_launch_rows(const bsim_dev *d, hipStream_t stream)
 
 extern "C" size_t bsim_step_lds_bytes(int32_t tile)
 {
-    (void)tile;
-    return LDS_FIXED;
+    return tile >= 2 * NWAVE * WAVE ? lds_bytes(Win<2>::RCAP) : lds_bytes(Win<1>::RCAP);
 }
 
 extern "C" hipError_t bsim_launch_step(const bsim_dev *d, int32_t t, int32_t flags,
