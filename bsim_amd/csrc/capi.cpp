@@ -325,11 +325,12 @@ extern "C" {
 
 /* Checks the simulation config and makes its device current (0, or a
  * failure code). */
-/* Tile schedule: the row-list kernel ranks the tiles heaviest first every
- * kRankEvery cycles (the volleys of the standard network move slowly against
- * 4 cycles: every 4th cycle keeps the gain of ranking at a quarter of its
- * cost); bsim_sim_set_schedule changes it. */
-static const int32_t kRankEvery = 4;
+/* Tile schedule: the row-list launch ranks the tiles heaviest first every
+ * kRankEvery cycles, by their work in the step launch before it (the volleys
+ * of the standard network move slowly against 2 cycles: on C1, every 2nd
+ * cycle ran 114.3 us per cycle, every cycle 114.8, every 4th 116.4);
+ * bsim_sim_set_schedule changes it. */
+static const int32_t kRankEvery = 2;
 
 static int32_t check_config(const bsim_sim_config &c, int32_t N, const char *who)
 {
@@ -514,9 +515,7 @@ static bsim_sim *sim_from_layout(bsim::Layout &L, const bsim_sim_config &c, bsim
     UP(src_hi, L.src_hi);
     UP(src_list_base, L.src_list_base);
     UP(chunk_tiles_off, L.chunk_tiles_off);
-    UP(chunk_tiles, L.chunk_tiles);
     UP(chunk_rng, L.chunk_rng);
-    UP(chunk_wrow, L.chunk_wrow);
     d.nsrc = (int32_t)L.src_lo.size();
     /* state */
     UP(v, L.v_init);

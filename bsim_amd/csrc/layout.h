@@ -141,18 +141,15 @@ typedef struct bsim_dev {
     uint32_t *rng_count;        /* [range entries] rows in the entry's chunk list this cycle */
     /* accounting / recording */
     int64_t *ev_count;          /* [ntiles] deliveries performed */
-    int32_t *clock;             /* [4]: cycle of the next launch, step workgroups finished,
-                                   row-list workgroups finished */
+    int32_t *clock;             /* [4]: cycle of the next launch, step workgroups finished */
     int32_t *perm;              /* [nslots] tile of each step workgroup (-1: none) */
-    uint64_t *tile_load;        /* [ntiles] expected work of the coming cycle (x 16) */
+    uint64_t *tile_load;        /* [ntiles] work of the launch before a ranking (x 16) */
     int32_t rank_every;         /* tiles ranked heaviest first every this many cycles (0: never,
                                    the tiles stay in their initial order) */
     int32_t pad5;
     uint8_t *tile_cls;          /* [ntiles] its half-octave class */
     const int64_t *chunk_tiles_off;  /* [nsrc + 1] tiles that read each chunk: */
-    const int32_t *chunk_tiles;      /*   tile ids */
     const int32_t *chunk_rng;        /*   the tile's range entry for the chunk */
-    const uint32_t *chunk_wrow;      /*   expected deliveries per active row, x 16 */
     uint64_t *raster;           /* [rec_steps][nwords] fired bitmaps, or NULL */
     int32_t rec_steps;
     int32_t rec_t0;             /* cycle of raster/trace row 0 */

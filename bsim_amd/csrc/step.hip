@@ -557,7 +557,7 @@ __device__ __forceinline__ void prod_stage(const Shared &S, const bsim_dev &d, i
 template <int S_>
 __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, int32_t nrb,
                               uint32_t qbase, uint32_t nsyn, const bsim_syn *syn,
-                              int64_t &events, Acc<S_> &acc)
+                              int64_t &events, uint32_t &rows, Acc<S_> &acc)
 {
     const uint32_t lane = threadIdx.x & (WAVE - 1);
     const uint32_t w = wave_index();
@@ -593,6 +593,7 @@ __device__ void deliver_batch(const Shared &S, const bsim_dev &d, int32_t r0, in
     __syncthreads();
     constexpr int RCAP = Win<S_>::RCAP, RPL = Win<S_>::RPL, RW = Win<S_>::RW;
     const uint32_t nrows_tot = uniform(S.cum[nrb]);
+    rows += nrows_tot;
     const int32_t nwin = (int32_t)((nrows_tot + RCAP - 1) / RCAP);
     if (nwin == 0) return;                                    /* uniform */
     /* The producer issues the list entries of the window after next (En)
@@ -695,11 +696,19 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
         const uint32_t nsyn = d.tile_nsyn[k];
         const bsim_syn *syn = d.syn + syn0;
         int64_t events = 0;
+        uint32_t rows = 0;
         const int32_t r0 = d.tile_rng[k], r1 = d.tile_rng[k + 1];
         for (int32_t rb = r0; rb < r1; rb += MAXR)
-            deliver_batch<S_>(S, d, rb, min(MAXR, r1 - rb), qbase, nsyn, syn, events, acc);
+            deliver_batch<S_>(S, d, rb, min(MAXR, r1 - rb), qbase, nsyn, syn, events, rows, acc);
         if (lane == 0 && w < NWAVE)
             atomicAdd((unsigned long long *)&d.ev_count[k], (unsigned long long)events);
+        /* the tile's work in this launch, when the row-list launch that
+         * follows ranks the tiles (it runs for cycle t + 1): its deliveries
+         * plus a fixed cost of two deliveries per active row, x 16 */
+        const int32_t re = d.rank_every;
+        if (clocked && (re == 1 || (re > 1 && (t + 1) % re == 0)) && lane == 0)
+            atomicAdd((unsigned long long *)&d.tile_load[k],
+                      w < NWAVE ? (unsigned long long)events * 16ull : (unsigned long long)rows * 32ull);
     }
 
     /* ---- phase (i) of cycle t: wavefront w updates its own quarter ---------- */
@@ -812,14 +821,14 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
     }
 }
 
-/* Orders the tiles by this cycle's expected work, heaviest first, so that the
- * hardware dispatches the heavy tiles before the light ones (results do not
- * depend on the order).  A tile's work (accumulated by the row-list
- * workgroups) is the active-row count of each of its source chunks weighted by
- * the tile's mean deliveries per row of that chunk plus a fixed cost of two
- * deliveries.  Counting sort over half-octave classes; within a class the
+/* Orders the tiles by their work in the step launch just finished, heaviest
+ * first, so that the hardware dispatches the heavy tiles of the coming launch
+ * before the light ones (results do not depend on the order; the firing
+ * volleys move slowly against one cycle).  A tile's work, accumulated by its
+ * own step workgroup, is its deliveries plus a fixed cost of two deliveries
+ * per active row.  Counting sort over half-octave classes; within a class the
  * order is arbitrary.  Runs in one workgroup of ROWS_THREADS threads, which
- * also clears the loads for the next cycle. */
+ * also clears the loads for the next ranking. */
 #define RANK_CLASSES 128
 
 __device__ __forceinline__ uint32_t load_class(uint64_t x)
@@ -924,6 +933,15 @@ __global__ __launch_bounds__(ROWS_THREADS) void bsim_rows_kernel(bsim_dev d)
 {
     __shared__ uint32_t scr[ROWS_THREADS / WAVE];
     const int32_t u = blockIdx.x;
+    /* one workgroup more than source chunks: it ranks the tiles every
+     * d.rank_every cycles (d.clock[0], the coming step launch's cycle, is the
+     * same for every workgroup of this launch), from the work the step
+     * launch just recorded, while the others build the lists */
+    if (u == d.nsrc) {
+        const bool rank = d.rank_every == 1 || (d.rank_every > 1 && d.clock[0] % d.rank_every == 0);
+        if (rank) rank_tiles(d);
+        return;
+    }
     const uint32_t tid = threadIdx.x;
     const uint32_t lane = tid & (WAVE - 1);
     const uint32_t w = tid / WAVE;
@@ -956,42 +974,17 @@ __global__ __launch_bounds__(ROWS_THREADS) void bsim_rows_kernel(bsim_dev d)
         const uint32_t rb = (uint32_t)(p0 + x - lo) * (uint32_t)D;
         for (uint64_t mm = m[x]; mm; mm &= mm - 1) out[pos++] = rb + (uint32_t)__builtin_ctzll(mm);
     }
-    /* the tiles are ranked every d.rank_every cycles (the order only affects
-     * speed); d.clock[0], the next step launch's cycle, is the same for every
-     * workgroup of this launch */
-    const bool rank = d.rank_every == 1 || (d.rank_every > 1 && d.clock[0] % d.rank_every == 0);
     /* for every tile that reads this chunk: the list's length in the tile's
-     * range entry and, when ranking, the tile's expected work -- rows
-     * weighted by the tile's mean deliveries per row plus a fixed cost of
-     * two */
-    for (int64_t x = d.chunk_tiles_off[u] + tid; x < d.chunk_tiles_off[u + 1]; x += ROWS_THREADS) {
+     * range entry */
+    for (int64_t x = d.chunk_tiles_off[u] + tid; x < d.chunk_tiles_off[u + 1]; x += ROWS_THREADS)
         d.rng_count[d.chunk_rng[x]] = tot;
-        if (rank)
-            atomicAdd((unsigned long long *)&d.tile_load[d.chunk_tiles[x]],
-                      (unsigned long long)tot * (unsigned long long)(d.chunk_wrow[x] + 32u));
-    }
-    if (!rank) return;
-
-    /* the last row-list workgroup to finish ranks the tiles for the coming
-     * step launch.  No fence: the ranking only orders the tiles, and any
-     * class assignment yields a permutation of all of them, so a load atomic
-     * still in flight can only make the order less good, never the result
-     * different (device-wide fences write back the L2 and cost more than the
-     * ranking gains). */
-    __shared__ uint32_t last;
-    __syncthreads();                /* this workgroup's atomics are issued */
-    if (tid == 0) last = (uint32_t)(atomicAdd(&d.clock[2], 1) == d.nsrc - 1);
-    __syncthreads();
-    if (!last) return;
-    rank_tiles(d);
-    if (tid == 0) d.clock[2] = 0;
 }
 
 extern "C" hipError_t bsim_launch_rows(const bsim_dev *d, hipStream_t stream)
 {
     if (d->nsrc <= 0) return hipSuccess;
     (void)hipGetLastError();       /* the check below then reports this launch alone */
-    hipLaunchKernelGGL(bsim_rows_kernel, dim3(d->nsrc), dim3(ROWS_THREADS), 0, stream, *d);
+    hipLaunchKernelGGL(bsim_rows_kernel, dim3(d->nsrc + 1), dim3(ROWS_THREADS), 0, stream, *d);
     return hipGetLastError();
 }
 

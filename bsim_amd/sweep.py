@@ -223,7 +223,7 @@ def run_solo(point, steps, tile=0, record=True):
     return raster, secs
 
 
-def autotune(points, steps=128, preroll=256, tiles=(192, 384, 768), ranks=(0, 1, 4)):
+def autotune(points, steps=128, preroll=256, tiles=(192, 384, 768), ranks=(0, 1, 2, 4)):
     """The project's schedule autotuner for a batch: the tile size (targets
     per workgroup) and the tile schedule (tiles ranked heaviest first every
     r cycles, 0 = never, Simulation.set_schedule) with the fastest run of
@@ -251,5 +251,5 @@ def autotune(points, steps=128, preroll=256, tiles=(192, 384, 768), ranks=(0, 1,
 
 def autotune_tile(points, steps=128, preroll=256, tiles=(192, 384, 768)):
     """The tile size alone (the default schedule): (best tile, {tile: seconds})."""
-    best, times = autotune(points, steps, preroll, tiles, ranks=(4,))
+    best, times = autotune(points, steps, preroll, tiles, ranks=(2,))
     return best[0], {t: s for (t, _), s in times.items()}
