@@ -62,7 +62,7 @@ typedef struct bsim_range {
     int64_t off_base;   /* off_tab index of the tile's vector of row (lo, 1) */
     int64_t lbase;      /* first entry of the chunk's active-row list in src_list */
     int32_t uid;        /* source chunk: index into src_* */
-    uint32_t wrow;      /* expected deliveries of one active row into the tile, x 16 */
+    uint32_t wrow;      /* expected deliveries of one active row into the tile, x 16 (host statistics) */
     uint32_t stride;    /* off_tab entries per (pre, delay) row of the group */
     uint32_t pad;
 } bsim_range;

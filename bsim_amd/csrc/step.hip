@@ -15,7 +15,8 @@
  *
  * bsim_rows_kernel then builds, for every source chunk, the list of its
  * active (pre, delay) rows for the next phase (ii), shared by all tiles that
- * receive from the chunk, and ranks the tiles by their expected work.
+ * receive from the chunk; one more workgroup ranks the tiles, heaviest first,
+ * by the work the step launch recorded.
  *
  * Phase (ii) for a tile, in windows of up to RCAP rows of the concatenated
  * lists of its source chunks:

@@ -218,9 +218,9 @@ void *bsim_sim_stream(bsim_sim *sim);
  * take the neurons and synapses in id order with a fixed grid,
  * SingleGPUSimulator.cu:100, 165-171): the order in which the
  * step's workgroups take the tiles.  Every rank_every cycles the row-list
- * kernel ranks the tiles by their expected deliveries, heaviest first (the
- * default, 4); 0 keeps the tiles in their initial order.  Results do not
- * depend on it.  Takes effect from the next step. */
+ * launch ranks the tiles by their work in the step launch before it,
+ * heaviest first (the default, 2); 0 keeps the tiles in their last order.
+ * Results do not depend on it.  Takes effect from the next step. */
 int32_t bsim_sim_set_schedule(bsim_sim *sim, int32_t rank_every);
 
 void bsim_sim_destroy(bsim_sim *sim);
