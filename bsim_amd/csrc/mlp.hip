@@ -168,14 +168,18 @@ __device__ inline void gemm_tile(const GemmArgs &g, int32_t bx, int32_t by, int3
         store_tile(As[0], a0, tid);
         store_tile(Bs[0], b0, tid);
     }
-    /* the epilogue's other operands, loaded while the products run: the
-     * activation mask (EPI_DRELU_BF16) or the targets (EPI_LOSS) of this
-     * lane's 16 outputs (rows m0 + 32 wm + 16 i + 4 (lane / 16) + r, column
-     * n0 + 32 wn + 16 j + lane % 16) */
+    /* the epilogue's other operands for this lane's 16 outputs (rows m0 +
+     * 32 wm + 16 i + 4 (lane / 16) + r, column n0 + 32 wn + 16 j + lane %
+     * 16): the activation mask (EPI_DRELU_BF16) is loaded while the products
+     * run; of the targets (EPI_LOSS) only the row indices are -- the target
+     * loads themselves are issued after the products, since loads issued
+     * before them hold up every wait of the k loop (the wait counter is in
+     * order) and the gathered targets are a chain of two loads */
     uint16_t mk[2][2][4];
     float yv[2][2][4];
-    if (EPI == EPI_DRELU_BF16 || EPI == EPI_LOSS) {
-        const uint32_t rq0 = (lane >> 4) * 4u, c16 = lane & 15u;
+    int32_t yr[2][4];
+    const uint32_t rq0 = (lane >> 4) * 4u, c16 = lane & 15u;
+    if (EPI == EPI_DRELU_BF16) {
 #pragma unroll
         for (int i = 0; i < 2; i++)
 #pragma unroll
@@ -184,13 +188,17 @@ __device__ inline void gemm_tile(const GemmArgs &g, int32_t bx, int32_t by, int3
                 for (int r = 0; r < 4; r++) {
                     const int32_t row = m0 + (int32_t)(wm * 32 + i * 16 + rq0) + r;
                     const int32_t col = n0 + (int32_t)(wn * 32 + j * 16 + c16);
-                    if (EPI == EPI_DRELU_BF16) {
-                        mk[i][j][r] = g.mask[(int64_t)row * g.ldo + col];
-                    } else {
-                        const int64_t yr = g.yrow ? (int64_t)g.yrow[row] : (int64_t)row;
-                        yv[i][j][r] = col < g.ny ? g.y[yr * g.ny + col] : 0.f;
-                    }
+                    mk[i][j][r] = g.mask[(int64_t)row * g.ldo + col];
                 }
+    }
+    if (EPI == EPI_LOSS) {
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int r = 0; r < 4; r++) {
+                const int32_t row = m0 + (int32_t)(wm * 32 + i * 16 + rq0) + r;
+                yr[i][r] = g.yrow ? g.yrow[row] : row;
+            }
     }
     __syncthreads();
     /* the first layer of a training step: A's rows are gathered, and the
@@ -231,6 +239,17 @@ __device__ inline void gemm_tile(const GemmArgs &g, int32_t bx, int32_t by, int3
         __syncthreads();
     }
 
+    if (EPI == EPI_LOSS) {
+#pragma unroll
+        for (int i = 0; i < 2; i++)
+#pragma unroll
+            for (int j = 0; j < 2; j++)
+#pragma unroll
+                for (int r = 0; r < 4; r++) {
+                    const int32_t col = n0 + (int32_t)(wn * 32 + j * 16 + c16);
+                    yv[i][j][r] = col < g.ny ? g.y[(int64_t)yr[i][r] * g.ny + col] : 0.f;
+                }
+    }
     /* lane holds C[row0 + 4 * (lane / 16) + r][col0 + lane % 16], r = 0..3 */
     const uint32_t rq = (lane >> 4) * 4u;
     float csum[2] = {0.f, 0.f};
@@ -342,7 +361,7 @@ struct AdamArgs {
     float lr, b1, b2, eps, c1, c2;   /* c1 = 1 - b1^t, c2 = 1 - b2^t */
     float *p, *m, *v, *grad;
     uint16_t *pbf;
-    const float *lpart;   /* loss partials of the step, summed by thread 0 */
+    const float *lpart;   /* loss partials of the step, summed by wavefront 0 of workgroup 0 */
     int32_t nlpart;
     float lscale;
     float *loss;
@@ -433,10 +452,18 @@ __global__ __launch_bounds__(ADAM_THREADS) void mlp_adam_tiles(AdamTileArgs t)
 {
     __shared__ uint16_t tile[64][66];
     const AdamArgs &a = t.a;
-    if (blockIdx.x == 0 && threadIdx.x == 0 && a.loss) {
+    if (blockIdx.x == 0 && threadIdx.x < WAVE && a.loss) {
+        /* the step's loss from the output tiles' partials: lane l adds
+         * partials l, l + 64, ... in order (their loads issued together),
+         * then a fixed butterfly combines the lanes; lane 0's total is
+         * written (one thread adding them one by one waited for each load
+         * in turn, which made this workgroup the last of the launch) */
         float s = 0.f;
-        for (int32_t k = 0; k < a.nlpart; k++) s += a.lpart[k];
-        *a.loss = s * a.lscale;
+#pragma unroll 4
+        for (int32_t k = (int32_t)threadIdx.x; k < a.nlpart; k += WAVE) s += a.lpart[k];
+#pragma unroll
+        for (int o = 1; o < WAVE; o <<= 1) s += __shfl_xor(s, o);
+        if (threadIdx.x == 0) *a.loss = s * a.lscale;
     }
     const int32_t b = (int32_t)blockIdx.x;
     if (b < t.wfirst[t.nlin]) {
