@@ -533,8 +533,9 @@ static bsim_sim *sim_from_layout(bsim::Layout &L, const bsim_sim_config &c, bsim
         if (!(d.active = dev_alloc<uint64_t>(sim, 2 * nw, e))) goto hip_fail;
         if (!(d.ev_count = dev_alloc<int64_t>(sim, nt, e))) goto hip_fail;
         if (!(sim->ev_fin = dev_alloc<int64_t>(sim, nt, e))) goto hip_fail;
-        if (!(d.clock = dev_alloc<int32_t>(sim, 4, e))) goto hip_fail;
-        if ((e = hipMemsetAsync(d.clock, 0, 4 * sizeof(int32_t), sim->stream)) != hipSuccess) goto hip_fail;
+        if (!(d.done = dev_alloc<uint64_t>(sim, 8 * BSIM_DONE_STRIDE, e))) goto hip_fail;
+        if ((e = hipMemsetAsync(d.done, 0, 8 * BSIM_DONE_STRIDE * sizeof(uint64_t), sim->stream)) != hipSuccess)
+            goto hip_fail;
         {
             /* until the first ranking: the tiles of each residue in order */
             std::vector<int32_t> perm((size_t)(L.nslots > 0 ? L.nslots : 1), -1);
@@ -547,6 +548,7 @@ static bsim_sim *sim_from_layout(bsim::Layout &L, const bsim_sim_config &c, bsim
             if (e != hipSuccess) goto hip_fail;
         }
         d.rank_every = kRankEvery;
+        d.done_inv = d.nslots >= 8 ? 8.0 / (double)d.nslots : 0.0;
         if (!(d.tile_load = dev_alloc<uint64_t>(sim, nt, e))) goto hip_fail;
         if ((e = hipMemsetAsync(d.tile_load, 0, nt * sizeof(uint64_t), sim->stream)) != hipSuccess) goto hip_fail;
         if (!(d.tile_cls = dev_alloc<uint8_t>(sim, nt, e))) goto hip_fail;

@@ -89,6 +89,7 @@ typedef struct __attribute__((aligned(16))) bsim_lifc {
 } bsim_lifc;
 
 #define BSIM_CLS_SET_MASK 0x3FFFFFFFu   /* ncls: kind << 30 | parameter set */
+#define BSIM_DONE_STRIDE 16              /* uint64 per completion counter: one 128-byte line each */
 
 /* Kernel arguments (passed by value). */
 typedef struct bsim_dev {
@@ -141,7 +142,10 @@ typedef struct bsim_dev {
     uint32_t *rng_count;        /* [range entries] rows in the entry's chunk list this cycle */
     /* accounting / recording */
     int64_t *ev_count;          /* [ntiles] deliveries performed */
-    int32_t *clock;             /* [4]: cycle of the next launch, step workgroups finished */
+    uint64_t *done;             /* [8 * BSIM_DONE_STRIDE]: step workgroups finished by clocked
+                                   launches, per XCD residue x at done[BSIM_DONE_STRIDE * x];
+                                   each grows by nslots / 8 per launch */
+    double done_inv;            /* 8 / nslots */
     int32_t *perm;              /* [nslots] tile of each step workgroup (-1: none) */
     uint64_t *tile_load;        /* [ntiles] work of the launch before a ranking (x 16) */
     int32_t rank_every;         /* tiles ranked heaviest first every this many cycles (0: never,

@@ -147,6 +147,30 @@ __device__ __forceinline__ void wave_sync()
 #define DPP_ROW_BCAST15 0x142
 #define DPP_ROW_BCAST31 0x143
 
+/* The device clock: the cycle of the coming clocked step launch.  Every
+ * workgroup of a clocked launch adds one to the completion counter of its XCD
+ * residue x = workgroup % 8 when it ends (an atomic whose result nobody waits
+ * for); each residue holds nslots / 8 workgroups, so its counter grows by
+ * that much per launch.  While any workgroup of launch t runs, its own
+ * residue's counter lies in [t nslots / 8, (t + 1) nslots / 8) -- it has not
+ * added its one yet -- so the quotient is t; a row-list launch, after its
+ * step launch has completed, reads t + 1 from any counter.  Plain (scalar,
+ * cached) loads: a launch sees the counts of the launches before it, and a
+ * value cached during the launch is still within its interval.  (A single
+ * counter whose last incrementer advanced the cycle made every workgroup wait
+ * on one contended address: 48 against 20 us per launch in a cycle without
+ * deliveries.) */
+__device__ __forceinline__ int32_t device_cycle(const bsim_dev &d, uint32_t x)
+{
+    /* count / (nslots / 8) without an integer division: the quotient's
+     * fractional part is a multiple of 8 / nslots, so half of that added
+     * before truncation absorbs the rounding of the product (below 2^-20 for
+     * cycles below 2^31, against a half step of at least 2^-19 for grids of
+     * up to 2^18 workgroups per residue) */
+    const double c = (double)d.done[BSIM_DONE_STRIDE * x];
+    return (int32_t)(c * d.done_inv + 0.5 * d.done_inv);
+}
+
 /* inclusive prefix sum over the wavefront: four in-row shifts, then the row
  * totals are carried across rows by two broadcasts; lanes without a source
  * read 0 */
@@ -642,7 +666,7 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
     extern __shared__ __align__(16) uint32_t lds[];
     const bool clocked = t < 0;
     if (clocked) {
-        t = d.clock[0];
+        t = device_cycle(d, blockIdx.x % 8u);
         flags = (t > 0 ? 1 : 0) | 2;
     }
     /* the heaviest tiles come first (bsim_rows_kernel ranks them): their
@@ -810,16 +834,9 @@ __global__ __launch_bounds__(NTHREAD) void bsim_step_kernel(bsim_dev d, int32_t 
             }
         }
     }
-    if (clocked) {
-        __syncthreads();            /* every wavefront has read d.clock[0] */
-        if (tid == 0) {
-            const int32_t done = atomicAdd(&d.clock[1], 1);
-            if (done == d.nslots - 1) {
-                d.clock[1] = 0;
-                d.clock[0] = t + 1;
-            }
-        }
-    }
+    if (clocked && tid == 0)        /* the cycle was read by every wavefront at its start */
+        (void)__hip_atomic_fetch_add(d.done + BSIM_DONE_STRIDE * (blockIdx.x % 8u), (uint64_t)1,
+                                     __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
 }
 
 /* Orders the tiles by their work in the step launch just finished, heaviest
@@ -934,12 +951,14 @@ __global__ __launch_bounds__(ROWS_THREADS) void bsim_rows_kernel(bsim_dev d)
 {
     __shared__ uint32_t scr[ROWS_THREADS / WAVE];
     const int32_t u = blockIdx.x;
+    /* the coming step launch's cycle (the step launch before this one has
+     * completed, so it is the same for every workgroup of this launch) */
+    const int32_t cyc = device_cycle(d, 0);
     /* one workgroup more than source chunks: it ranks the tiles every
-     * d.rank_every cycles (d.clock[0], the coming step launch's cycle, is the
-     * same for every workgroup of this launch), from the work the step
-     * launch just recorded, while the others build the lists */
+     * d.rank_every cycles, from the work the step launch just recorded,
+     * while the others build the lists */
     if (u == d.nsrc) {
-        const bool rank = d.rank_every == 1 || (d.rank_every > 1 && d.clock[0] % d.rank_every == 0);
+        const bool rank = d.rank_every == 1 || (d.rank_every > 1 && cyc % d.rank_every == 0);
         if (rank) rank_tiles(d);
         return;
     }
@@ -947,7 +966,7 @@ __global__ __launch_bounds__(ROWS_THREADS) void bsim_rows_kernel(bsim_dev d)
     const uint32_t lane = tid & (WAVE - 1);
     const uint32_t w = tid / WAVE;
     const int32_t lo = d.src_lo[u], hi = d.src_hi[u];
-    const int32_t par = (d.clock[0] - 1) & 1;      /* parity of the last update */
+    const int32_t par = (cyc - 1) & 1;             /* parity of the last update */
     const uint64_t *hist = d.hist + (int64_t)par * d.n;
     uint32_t *out = d.src_list + d.src_list_base[u];
     const int32_t D = d.dmax;
