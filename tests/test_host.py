@@ -226,6 +226,33 @@ def test_run_cycles_rounds_halves_away_from_zero():
         int(libm.roundf(float(np.float32(np.float32(4.5) * dt) / dt)))
 
 
+def test_device_cycle_quotient_is_exact():
+    """The step kernel reads the cycle as count / (nslots / 8) from a residue's
+    completion counter without an integer division: truncate(count * inv +
+    inv / 2) with inv = 8 / nslots in float64, as the library computes it on
+    the host (csrc/capi.cpp) and the kernel on the device (csrc/step.hip,
+    compiled without contraction, so the product and the sum round
+    separately, as numpy's do).  Exact for every count of the form t * per +
+    r, 0 <= r < per, over the grids and cycles the kernel allows (per up to
+    2^18 - 1 workgroups per residue, t up to 2^31 - 1), checked at both ends
+    of each interval and inside it."""
+    rng = np.random.default_rng(11)
+    pers = [1, 2, 3, 7, 8, 171, 438, 439, 1000, 4096, 12711, 65537, (1 << 17) + 3, (1 << 18) - 1]
+    ts = np.concatenate([np.arange(0, 64), [2**31 - 1, 2**31 - 2, 2**30, 2**24 + 1],
+                         rng.integers(0, 2**31 - 1, 2000)]).astype(np.int64)
+    for per in pers:
+        nslots = 8 * per
+        inv = np.float64(8.0) / np.float64(nslots)
+        half = np.float64(0.5) * inv
+        rs = np.unique(np.array([0, 1, per // 2, max(per - 2, 0), per - 1], dtype=np.int64))
+        rs = rs[rs < per]
+        c = (ts[:, None] * per + rs[None, :]).ravel()
+        assert c.max() < 2**53
+        got = np.trunc(c.astype(np.float64) * inv + half).astype(np.int64)
+        want = np.repeat(ts, len(rs))
+        assert np.array_equal(got, want), per
+
+
 def test_host_layout_check_under_sanitizers():
     """tests/cpp/host_layout_check.cpp, built from the library's host sources
     under AddressSanitizer and UBSan: networks laid out in parts and joined
