@@ -12,14 +12,24 @@ SCAN = ["oracle", "include", os.path.join("bsim_amd", "csrc"), "bsim_amd", "test
         "bench.py", "__graft_entry__.py", "DESIGN.md", "INTEGRATION.md"]
 EXT = (".c", ".cpp", ".h", ".hip", ".inc", ".py", ".md", ".sh")
 CITE = re.compile(r"((?:[\w.-]+/)*[\w.-]+\.(?:cpp|cu|h|py|c|txt|std|cmake|md)):(\d+)(?:-(\d+))?")
-# this repository's own files, cited by path
-OWN = ("bsim_amd/", "oracle/", "tests/", "include/bsim_amd.h", "include/bsim_mlp.h",
-       "csrc/", "bench.py", "DESIGN.md", "SURVEY.md", "BASELINE.md", "INTEGRATION.md",
-       "VERDICT.md", "ADVICE.md", "README.md", "layout.h", "step.hip", "capi.cpp",
-       "network.cpp", "network.h", "gen.cpp", "mlp.hip", "api.py", "netspec.py", "sweep.py",
-       "surrogate.py", "stsim.c", "stsim_body.inc", "bsim_oracle.h", "mlp_oracle.py",
-       "oracle.py", "ref_io.py", "ref_harness.cpp", "make_golden.py", "golden_io.py",
-       "test_", "conftest.py", "dropin_run.cpp", "standard_run.cpp", "__graft_entry__.py")
+
+
+def own_files():
+    """This repository's files: repository-relative paths and file names."""
+    names, paths = set(), set()
+    for dp, dn, fn in os.walk(ROOT):
+        dn[:] = [d for d in dn if d not in (".git", "_work", "gpurun_out", "__pycache__")]
+        for f in fn:
+            names.add(f)
+            paths.add(os.path.relpath(os.path.join(dp, f), ROOT).replace(os.sep, "/"))
+    return names, paths
+
+
+def is_own(path, names, paths):
+    """A cited path that names a file of this repository (by its path, a tail
+    of it, or its file name)."""
+    return (path in paths or os.path.basename(path) in names or
+            any(p.endswith("/" + path) for p in paths))
 
 
 def files():
@@ -59,13 +69,14 @@ def resolve(path, idx):
 
 def citations():
     out = []
+    names, paths = own_files()
     for fn in sorted(set(files())):
         if os.path.abspath(fn) == os.path.abspath(__file__):
             continue
         for ln, line in enumerate(open(fn, errors="replace"), 1):
             for m in CITE.finditer(line):
                 path = m.group(1)
-                if path.startswith(OWN) or os.path.basename(path).startswith(OWN):
+                if is_own(path, names, paths):
                     continue
                 out.append((os.path.relpath(fn, ROOT), ln, path, int(m.group(2)),
                             int(m.group(3) or m.group(2))))
